@@ -1,0 +1,96 @@
+/*
+ * socu_hip.h -- C ABI of the MI355X (gfx950) nested-dissection block-tridiagonal
+ * Cholesky library, libsocu_hip.so.
+ *
+ * The reference (PREDICT-EPFL/socu) is pure Python over NVIDIA Warp; its
+ * "native boundary" is the set of launch factories in
+ * socu/block_tridiag_solver.py.  Each entry point below replaces the device
+ * work one of those factories enqueues, so the Python mirror in
+ * socu_amd/block_tridiag_solver.py can keep the reference's signatures.
+ *
+ * Conventions (all entry points):
+ *  - L, E, x are device pointers to contiguous row-major arrays
+ *      L (B, N, n, n), E (B, n_E, n, n) with n_E = socu_calculate_off_diag_storage_len(N),
+ *      x (B, N, n, m);
+ *    dtype selects the element type (SOCU_DTYPE_F32 = float, SOCU_DTYPE_F64 = double).
+ *  - mask is a device pointer to B bytes or NULL; a zero byte leaves that
+ *    batch entry's L, E and x untouched (README.md:214-235).  It is read by
+ *    the kernels on every execution, including graph replays.
+ *  - stream is a hipStream_t (NULL = the null stream).  Work is enqueued
+ *    asynchronously; nothing synchronises the host.  No entry point allocates
+ *    device memory.
+ *  - Return 0 on success, a negative SOCU_E* code otherwise;
+ *    socu_last_error() returns the calling thread's last message.
+ */
+#ifndef SOCU_HIP_H
+#define SOCU_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SOCU_DTYPE_F32 0
+#define SOCU_DTYPE_F64 1
+
+#define SOCU_OK 0
+#define SOCU_EINVAL -1      /* bad argument (shape, dtype, pointer) */
+#define SOCU_EUNSUPPORTED -2 /* valid but not supported (e.g. n > max block size) */
+#define SOCU_EHIP -3        /* a HIP runtime call failed */
+
+/* Operations a plan can hold (the five launch factories). */
+#define SOCU_OP_FACTOR 0           /* create_cholesky_factor_launch, block_tridiag_solver.py:739-923 */
+#define SOCU_OP_SOLVE 1            /* create_cholesky_solve_launch, :1520-1727 */
+#define SOCU_OP_FACTOR_AND_SOLVE 2 /* create_cholesky_factor_and_solve_launch, :926-1233 */
+#define SOCU_OP_FORWARD 3          /* create_cholesky_forward_substitution_launch, :1236-1374 */
+#define SOCU_OP_BACKWARD 4         /* create_cholesky_backward_substitution_launch, :1377-1517 */
+
+/* calculate_recursive_iterations, block_tridiag_solver.py:12-15 (N >= 1). */
+int socu_calculate_recursive_iterations(int N);
+
+/* calculate_off_diag_storage_len, block_tridiag_solver.py:18-23 (N >= 1). */
+long long socu_calculate_off_diag_storage_len(int N);
+
+/* Largest block size n the kernels accept. */
+int socu_max_block_size(void);
+
+/* In-place factor of every unmasked system: tril(L[b,i]) <- diagonal Cholesky
+ * blocks, E[b] <- ND off-diagonal factor blocks including fill-in.
+ * Replaces the device work of create_cholesky_factor_launch's callback
+ * (block_tridiag_solver.py:864-918).  E[b, N-1:] must be zero on entry. */
+int socu_cholesky_factor(void* L, void* E, const unsigned char* mask, int B, int N, int n, int dtype, void* stream);
+
+/* x <- Lp^{-1} P x (forward sweep), block_tridiag_solver.py:1339-1369. */
+int socu_cholesky_forward_substitution(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N,
+                                       int n, int m, int dtype, void* stream);
+
+/* x <- P^T Lp^{-T} x (backward sweep), block_tridiag_solver.py:1480-1512. */
+int socu_cholesky_backward_substitution(const void* L, const void* E, void* x, const unsigned char* mask, int B,
+                                        int N, int n, int m, int dtype, void* stream);
+
+/* forward then backward sweep, block_tridiag_solver.py:1666-1722. */
+int socu_cholesky_solve(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
+                        int dtype, void* stream);
+
+/* factor, forward, backward, block_tridiag_solver.py:1124-1228. */
+int socu_cholesky_factor_and_solve(void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
+                                   int dtype, void* stream);
+
+/* A plan binds one operation to its arrays, like the reference's launch
+ * callables (arrays bound at creation).  With use_graph != 0 the work is
+ * captured once into a hipGraph on a private capture stream and replayed by
+ * socu_plan_execute (replaces utils.py:40-66, create_cuda_graph_callback).
+ * x may be NULL for SOCU_OP_FACTOR. */
+typedef struct socu_plan socu_plan;
+int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N,
+                     int n, int m, int dtype, int use_graph);
+int socu_plan_execute(socu_plan* plan, void* stream);
+int socu_plan_destroy(socu_plan* plan);
+
+/* Last error message of the calling thread ("" if none). */
+const char* socu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOCU_HIP_H */

@@ -1,0 +1,89 @@
+"""ctypes binding of libsocu_hip.so (the C ABI declared in include/socu_hip.h).
+
+torch is imported before the library is opened: torch ships a HIP runtime
+whose SONAME (libamdhip64.so.7) is the one libsocu_hip.so needs, so loading in
+this order makes both share one runtime, one device context and one set of
+streams.  There is no fallback: if the library is missing the import fails.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (must be loaded first, see module docstring)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsocu_hip.so")
+
+DTYPE_F32 = 0
+DTYPE_F64 = 1
+
+OK = 0
+EINVAL = -1
+EUNSUPPORTED = -2
+EHIP = -3
+
+OP_FACTOR = 0
+OP_SOLVE = 1
+OP_FACTOR_AND_SOLVE = 2
+OP_FORWARD = 3
+OP_BACKWARD = 4
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+
+# name -> (restype, argtypes); every symbol declared in include/socu_hip.h
+SIGNATURES = {
+    "socu_calculate_recursive_iterations": (_i, [_i]),
+    "socu_calculate_off_diag_storage_len": (ctypes.c_longlong, [_i]),
+    "socu_max_block_size": (_i, []),
+    "socu_cholesky_factor": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _vp]),
+    "socu_cholesky_forward_substitution": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "socu_cholesky_backward_substitution": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "socu_cholesky_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "socu_cholesky_factor_and_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "socu_plan_create": (_i, [ctypes.POINTER(_vp), _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i]),
+    "socu_plan_execute": (_i, [_vp, _vp]),
+    "socu_plan_destroy": (_i, [_vp]),
+    "socu_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+class SocuError(RuntimeError):
+    """A call into libsocu_hip.so returned an error code."""
+
+
+def load():
+    """Open libsocu_hip.so once and bind every C-ABI symbol."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"socu_amd: {LIB_PATH} is missing; build it with `python -c \"import __graft_entry__ as g; g.build()\"` "
+            "(hipcc --offload-arch=gfx950).  There is no CPU fallback."
+        )
+    lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    msg = load().socu_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str):
+    if rc != OK:
+        msg = last_error()
+        if rc == EINVAL:
+            raise ValueError(f"{what}: {msg}")
+        if rc == EUNSUPPORTED:
+            raise NotImplementedError(f"{what}: {msg}")
+        raise SocuError(f"{what}: {msg} (code {rc})")

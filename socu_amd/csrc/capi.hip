@@ -1,0 +1,255 @@
+// C-ABI of libsocu_hip.so: argument validation, the ND level loops, plans and
+// hipGraph capture.  The level loops here replace the reference's host
+// callbacks (block_tridiag_solver.py:792-813, :864-918, :992-1028,
+// :1124-1228, :1294-1310, :1339-1369, :1435-1451, :1480-1512, :1586-1616,
+// :1666-1722); no Python runs per level.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/socu_hip.h"
+#include "nd_common.h"
+#include "nd_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(SOCU_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int check_dims(const void* L, const void* E, const void* x, bool need_x, int B, int N, int n, int m, int dtype) {
+  if (dtype != SOCU_DTYPE_F32 && dtype != SOCU_DTYPE_F64) return fail(SOCU_EINVAL, "dtype must be f32 or f64");
+  if (B < 0) return fail(SOCU_EINVAL, "batch dimension must be >= 0 (got %d)", B);
+  if (N < 1) return fail(SOCU_EINVAL, "horizon N must be >= 1 (got %d)", N);
+  if (n < 1) return fail(SOCU_EINVAL, "block size n must be >= 1 (got %d)", n);
+  if (n > socu::kMaxN)
+    return fail(SOCU_EUNSUPPORTED, "block size n=%d exceeds the supported maximum %d", n, socu::kMaxN);
+  if (need_x && m < 1) return fail(SOCU_EINVAL, "number of right-hand sides m must be >= 1 (got %d)", m);
+  if (B > 0) {
+    if (L == nullptr) return fail(SOCU_EINVAL, "L is NULL");
+    if (E == nullptr && N > 1) return fail(SOCU_EINVAL, "E is NULL");
+    if (need_x && x == nullptr) return fail(SOCU_EINVAL, "x is NULL");
+  }
+  if (B > 65535) return fail(SOCU_EUNSUPPORTED, "batch dimension %d exceeds 65535", B);
+  return SOCU_OK;
+}
+
+template <typename T>
+int run_factor(T* L, T* E, const uint8_t* mask, int B, int N, int n, hipStream_t st) {
+  const int Lv = socu::recursive_iterations(N);
+  for (int l = 0; l < Lv; ++l) {
+    hipError_t e = socu::factor_level<T>(L, E, mask, B, N, n, l, st);
+    if (e != hipSuccess) return hip_fail(e, "factor level launch");
+  }
+  return SOCU_OK;
+}
+
+template <typename T>
+int run_forward(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m, hipStream_t st) {
+  const int Lv = socu::recursive_iterations(N);
+  for (int step = -1; step < Lv - 1; ++step) {
+    hipError_t e = socu::forward_step<T>(L, E, x, mask, B, N, n, m, step, st);
+    if (e != hipSuccess) return hip_fail(e, "forward step launch");
+  }
+  return SOCU_OK;
+}
+
+template <typename T>
+int run_backward(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m, hipStream_t st) {
+  const int Lv = socu::recursive_iterations(N);
+  for (int l = Lv - 1; l >= 0; --l) {
+    hipError_t e = socu::backward_level<T>(L, E, x, mask, B, N, n, m, l, st);
+    if (e != hipSuccess) return hip_fail(e, "backward level launch");
+  }
+  return SOCU_OK;
+}
+
+template <typename T>
+int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N, int n, int m, hipStream_t st) {
+  T* Lt = static_cast<T*>(L);
+  T* Et = static_cast<T*>(E);
+  T* xt = static_cast<T*>(x);
+  int rc = SOCU_OK;
+  switch (op) {
+    case SOCU_OP_FACTOR:
+      return run_factor<T>(Lt, Et, mask, B, N, n, st);
+    case SOCU_OP_FORWARD:
+      return run_forward<T>(Lt, Et, xt, mask, B, N, n, m, st);
+    case SOCU_OP_BACKWARD:
+      return run_backward<T>(Lt, Et, xt, mask, B, N, n, m, st);
+    case SOCU_OP_SOLVE:
+      rc = run_forward<T>(Lt, Et, xt, mask, B, N, n, m, st);
+      if (rc) return rc;
+      return run_backward<T>(Lt, Et, xt, mask, B, N, n, m, st);
+    case SOCU_OP_FACTOR_AND_SOLVE:
+      rc = run_factor<T>(Lt, Et, mask, B, N, n, st);
+      if (rc) return rc;
+      rc = run_forward<T>(Lt, Et, xt, mask, B, N, n, m, st);
+      if (rc) return rc;
+      return run_backward<T>(Lt, Et, xt, mask, B, N, n, m, st);
+    default:
+      return fail(SOCU_EINVAL, "unknown operation %d", op);
+  }
+}
+
+// Runs the per-process kernel set-up once; returns SOCU_OK or an error code.
+int ensure_init() {
+  static int state = 0;  // 0 = not yet, 1 = done
+  if (state == 1) return SOCU_OK;
+  hipError_t e = socu::init_factor_kernels();
+  if (e == hipSuccess) e = socu::init_solve_kernels();
+  if (e != hipSuccess) return hip_fail(e, "kernel set-up (hipFuncSetAttribute)");
+  state = 1;
+  return SOCU_OK;
+}
+
+int dispatch(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m, int dtype,
+             void* stream) {
+  const bool need_x = op != SOCU_OP_FACTOR;
+  int rc = check_dims(L, E, x, need_x, B, N, n, m, dtype);
+  if (rc) return rc;
+  if (B == 0) return SOCU_OK;
+  rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == SOCU_DTYPE_F64) return run_op<double>(op, L, E, x, mask, B, N, n, m, st);
+  return run_op<float>(op, L, E, x, mask, B, N, n, m, st);
+}
+
+}  // namespace
+
+struct socu_plan {
+  int op;
+  void* L;
+  void* E;
+  void* x;
+  const unsigned char* mask;
+  int B, N, n, m, dtype;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" {
+
+int socu_calculate_recursive_iterations(int N) {
+  if (N < 1) return fail(SOCU_EINVAL, "horizon must be >= 1");
+  return socu::recursive_iterations(N);
+}
+
+long long socu_calculate_off_diag_storage_len(int N) {
+  if (N < 1) return fail(SOCU_EINVAL, "horizon must be >= 1");
+  return socu::off_diag_storage_len(N);
+}
+
+int socu_max_block_size(void) { return socu::kMaxN; }
+
+const char* socu_last_error(void) { return g_last_error.c_str(); }
+
+int socu_cholesky_factor(void* L, void* E, const unsigned char* mask, int B, int N, int n, int dtype, void* stream) {
+  return dispatch(SOCU_OP_FACTOR, L, E, nullptr, mask, B, N, n, 1, dtype, stream);
+}
+
+int socu_cholesky_forward_substitution(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N,
+                                       int n, int m, int dtype, void* stream) {
+  return dispatch(SOCU_OP_FORWARD, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream);
+}
+
+int socu_cholesky_backward_substitution(const void* L, const void* E, void* x, const unsigned char* mask, int B,
+                                        int N, int n, int m, int dtype, void* stream) {
+  return dispatch(SOCU_OP_BACKWARD, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream);
+}
+
+int socu_cholesky_solve(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
+                        int dtype, void* stream) {
+  return dispatch(SOCU_OP_SOLVE, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream);
+}
+
+int socu_cholesky_factor_and_solve(void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
+                                   int dtype, void* stream) {
+  return dispatch(SOCU_OP_FACTOR_AND_SOLVE, L, E, x, mask, B, N, n, m, dtype, stream);
+}
+
+int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N,
+                     int n, int m, int dtype, int use_graph) {
+  if (out == nullptr) return fail(SOCU_EINVAL, "plan output pointer is NULL");
+  *out = nullptr;
+  if (op < SOCU_OP_FACTOR || op > SOCU_OP_BACKWARD) return fail(SOCU_EINVAL, "unknown operation %d", op);
+  const bool need_x = op != SOCU_OP_FACTOR;
+  int rc = check_dims(L, E, x, need_x, B, N, n, need_x ? m : 1, dtype);
+  if (rc) return rc;
+  if (B > 0) {
+    rc = ensure_init();
+    if (rc) return rc;
+  }
+  socu_plan* p = new socu_plan{op, L, E, x, mask, B, N, n, need_x ? m : 1, dtype, nullptr, nullptr};
+  if (use_graph && B > 0) {
+    hipStream_t cs;
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete p;
+      return hip_fail(e, "hipStreamCreate (graph capture)");
+    }
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      (void)hipStreamDestroy(cs);
+      delete p;
+      return hip_fail(e, "hipStreamBeginCapture");
+    }
+    rc = dispatch(op, L, E, x, mask, B, N, n, p->m, dtype, cs);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (rc != SOCU_OK || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      delete p;
+      return rc != SOCU_OK ? rc : hip_fail(e2, "hipStreamEndCapture");
+    }
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      delete p;
+      return hip_fail(e, "hipGraphInstantiate");
+    }
+    p->graph = g;
+    p->exec = ex;
+  }
+  *out = p;
+  return SOCU_OK;
+}
+
+int socu_plan_execute(socu_plan* p, void* stream) {
+  if (p == nullptr) return fail(SOCU_EINVAL, "plan is NULL");
+  if (p->B == 0) return SOCU_OK;
+  if (p->exec) {
+    hipError_t e = hipGraphLaunch(p->exec, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    return SOCU_OK;
+  }
+  return dispatch(p->op, p->L, p->E, p->x, p->mask, p->B, p->N, p->n, p->m, p->dtype, stream);
+}
+
+int socu_plan_destroy(socu_plan* p) {
+  if (p == nullptr) return SOCU_OK;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  delete p;
+  return SOCU_OK;
+}
+
+}  // extern "C"

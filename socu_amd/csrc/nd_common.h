@@ -1,0 +1,85 @@
+// Shared definitions for the nested-dissection (ND) block-tridiagonal
+// Cholesky kernels on gfx950.
+//
+// Data layout in HBM (identical to socu's, README.md:54-80):
+//   L  (B, N,   n, n) row-major, diagonal blocks; only tril is read.
+//   E  (B, n_E, n, n) row-major, level segments of the ND fill-in storage;
+//      segment l starts at off_l = sum_{l'<l} (N >> l') - 1 and entry k couples
+//      column (k+1)s-1 with row (k+2)s-1, s = 2^l
+//      (block_tridiag_solver.py:18-23, :793-813).
+//   x  (B, N,   n, m) row-major right-hand sides / solutions.
+//   mask (B,) bytes, optional; a zero entry leaves that system untouched.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace socu {
+
+// Threads per workgroup for every level kernel: four 64-wide wavefronts.
+constexpr int kThreads = 256;
+
+// Largest block size n handled by the LDS-resident kernels.  A packed
+// lower triangle of a 128x128 fp64 block is 66 KiB of LDS.
+constexpr int kMaxN = 128;
+
+// Column chunk of the right-hand sides processed by one workgroup.
+constexpr int kRhsChunk = 16;
+
+// Output tile edge of the Schur-complement kernel.
+constexpr int kTile = 32;
+
+// Column/row chunk of a triangular-solve workgroup in the factor.
+constexpr int kTrsmChunk = 32;
+
+// K-chunk of the LDS-staged block products.
+constexpr int kKChunk = 32;
+
+struct Level {
+  int l;         // level index
+  int s;         // stride 2^l
+  int dim;       // nodes at this level: i = s-1+2*s*t, t < dim
+  int off;       // first E index of this level's segment
+  int next_off;  // first E index of level l+1's segment
+};
+
+__host__ __device__ inline int recursive_iterations(int N) {
+  int L = 0;
+  while (N > 0) { ++L; N >>= 1; }
+  return L;
+}
+
+__host__ __device__ inline long long off_diag_storage_len(int N) {
+  long long t = 0;
+  const int L = recursive_iterations(N);
+  for (int l = 0; l < L; ++l) t += (N >> l) - 1;
+  return t;
+}
+
+__host__ inline Level level_info(int N, int l) {
+  Level v;
+  v.l = l;
+  v.s = 1 << l;
+  v.dim = (N + v.s) / (2 * v.s);
+  int off = 0;
+  for (int q = 0; q < l; ++q) off += (N >> q) - 1;
+  v.off = off;
+  v.next_off = off + (N >> l) - 1;
+  return v;
+}
+
+// Index of the first element of a packed lower-triangular row r.
+__device__ __forceinline__ int tri(int r) { return (r * (r + 1)) >> 1; }
+
+__device__ __forceinline__ float rsqrt_t(float x) { return 1.0f / sqrtf(x); }
+__device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
+__device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
+
+// True when batch entry b is switched off by the mask.
+__device__ __forceinline__ bool masked_out(const uint8_t* mask, int b) {
+  return mask != nullptr && mask[b] == 0;
+}
+
+}  // namespace socu

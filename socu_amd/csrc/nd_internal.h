@@ -1,0 +1,30 @@
+// Host-side launchers shared between the kernel files and the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace socu {
+
+// One-time per-process kernel set-up (LDS limits above 64 KiB).  Must run
+// before any stream capture.
+hipError_t init_factor_kernels();
+hipError_t init_solve_kernels();
+
+// Factor kernels of one ND level (potrf, then trsm and Schur updates unless
+// it is the last level).
+template <typename T>
+hipError_t factor_level(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, hipStream_t st);
+
+// Forward-substitution step: step -1 solves level 0; step l >= 0 applies
+// level l's updates to every receiving block and solves level l+1.
+template <typename T>
+hipError_t forward_step(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m, int step,
+                        hipStream_t st);
+
+// Backward-substitution kernel of one ND level.
+template <typename T>
+hipError_t backward_level(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m, int lidx,
+                          hipStream_t st);
+
+}  // namespace socu
