@@ -86,6 +86,16 @@ int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const u
 int socu_plan_execute(socu_plan* plan, void* stream);
 int socu_plan_destroy(socu_plan* plan);
 
+/* Instrumentation (no counterpart in the reference): runs `op` once like the
+ * direct entry points, with a hipEvent pair around every kernel launch on
+ * `stream`, synchronises `stream`, and returns per kernel class the summed
+ * elapsed milliseconds and the launch count.  Classes: 0 potrf, 1 trsm,
+ * 2 Schur update, 3 forward step, 4 backward level.  Creates events, so it is
+ * for measurement, not for hot loops. */
+#define SOCU_KERNEL_CLASSES 5
+int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
+                    int dtype, void* stream, double* kernel_ms, int* kernel_launches);
+
 /* Last error message of the calling thread ("" if none). */
 const char* socu_last_error(void);
 

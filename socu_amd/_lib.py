@@ -29,6 +29,8 @@ OP_FACTOR_AND_SOLVE = 2
 OP_FORWARD = 3
 OP_BACKWARD = 4
 
+KERNEL_CLASSES = ("potrf", "trsm", "schur", "forward", "backward")
+
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 
@@ -45,6 +47,8 @@ SIGNATURES = {
     "socu_plan_create": (_i, [ctypes.POINTER(_vp), _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i]),
     "socu_plan_execute": (_i, [_vp, _vp]),
     "socu_plan_destroy": (_i, [_vp]),
+    "socu_profile_op": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp,
+                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
     "socu_last_error": (ctypes.c_char_p, []),
 }
 

@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/socu_hip.h"
 #include "nd_common.h"
@@ -53,7 +54,7 @@ template <typename T>
 int run_factor(T* L, T* E, const uint8_t* mask, int B, int N, int n, hipStream_t st) {
   const int Lv = socu::recursive_iterations(N);
   for (int l = 0; l < Lv; ++l) {
-    hipError_t e = socu::factor_level<T>(L, E, mask, B, N, n, l, st);
+    hipError_t e = socu::factor_level_blocked<T>(L, E, mask, B, N, n, l, st);
     if (e != hipSuccess) return hip_fail(e, "factor level launch");
   }
   return SOCU_OK;
@@ -63,7 +64,7 @@ template <typename T>
 int run_forward(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m, hipStream_t st) {
   const int Lv = socu::recursive_iterations(N);
   for (int step = -1; step < Lv - 1; ++step) {
-    hipError_t e = socu::forward_step<T>(L, E, x, mask, B, N, n, m, step, st);
+    hipError_t e = socu::forward_step_blocked<T>(L, E, x, mask, B, N, n, m, step, st);
     if (e != hipSuccess) return hip_fail(e, "forward step launch");
   }
   return SOCU_OK;
@@ -73,7 +74,7 @@ template <typename T>
 int run_backward(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m, hipStream_t st) {
   const int Lv = socu::recursive_iterations(N);
   for (int l = Lv - 1; l >= 0; --l) {
-    hipError_t e = socu::backward_level<T>(L, E, x, mask, B, N, n, m, l, st);
+    hipError_t e = socu::backward_level_blocked<T>(L, E, x, mask, B, N, n, m, l, st);
     if (e != hipSuccess) return hip_fail(e, "backward level launch");
   }
   return SOCU_OK;
@@ -113,6 +114,7 @@ int ensure_init() {
   if (state == 1) return SOCU_OK;
   hipError_t e = socu::init_factor_kernels();
   if (e == hipSuccess) e = socu::init_solve_kernels();
+  if (e == hipSuccess) e = socu::init_blocked_kernels();
   if (e != hipSuccess) return hip_fail(e, "kernel set-up (hipFuncSetAttribute)");
   state = 1;
   return SOCU_OK;
@@ -131,7 +133,36 @@ int dispatch(int op, void* L, void* E, void* x, const unsigned char* mask, int B
   return run_op<float>(op, L, E, x, mask, B, N, n, m, st);
 }
 
+// Per-thread launch timer used by socu_profile_op.
+struct TimerState {
+  bool active = false;
+  std::vector<int> cls;
+  std::vector<hipEvent_t> ev;  // begin/end pairs
+  int open = -1;
+};
+thread_local TimerState g_timer;
+
 }  // namespace
+
+namespace socu {
+
+void kt_begin(int cls, hipStream_t st) {
+  if (!g_timer.active) return;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+  (void)hipEventRecord(a, st);
+  g_timer.cls.push_back(cls);
+  g_timer.ev.push_back(a);
+  g_timer.ev.push_back(b);
+}
+
+void kt_end(int cls, hipStream_t st) {
+  (void)cls;
+  if (!g_timer.active || g_timer.ev.empty()) return;
+  (void)hipEventRecord(g_timer.ev.back(), st);
+}
+
+}  // namespace socu
 
 struct socu_plan {
   int op;
@@ -182,6 +213,33 @@ int socu_cholesky_solve(const void* L, const void* E, void* x, const unsigned ch
 int socu_cholesky_factor_and_solve(void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
                                    int dtype, void* stream) {
   return dispatch(SOCU_OP_FACTOR_AND_SOLVE, L, E, x, mask, B, N, n, m, dtype, stream);
+}
+
+int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
+                    int dtype, void* stream, double* kernel_ms, int* kernel_launches) {
+  if (kernel_ms == nullptr || kernel_launches == nullptr) return fail(SOCU_EINVAL, "output arrays are NULL");
+  for (int c = 0; c < socu::kKcCount; ++c) {
+    kernel_ms[c] = 0.0;
+    kernel_launches[c] = 0;
+  }
+  g_timer = TimerState();
+  g_timer.active = true;
+  int rc = dispatch(op, L, E, x, mask, B, N, n, m, dtype, stream);
+  g_timer.active = false;
+  hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+  for (size_t q = 0; q < g_timer.cls.size(); ++q) {
+    float ms = 0.f;
+    if (rc == SOCU_OK && e == hipSuccess &&
+        hipEventElapsedTime(&ms, g_timer.ev[2 * q], g_timer.ev[2 * q + 1]) == hipSuccess) {
+      kernel_ms[g_timer.cls[q]] += ms;
+      kernel_launches[g_timer.cls[q]] += 1;
+    }
+  }
+  for (hipEvent_t ev : g_timer.ev) (void)hipEventDestroy(ev);
+  g_timer = TimerState();
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize (profile)");
+  return SOCU_OK;
 }
 
 int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N,

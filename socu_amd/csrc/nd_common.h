@@ -72,8 +72,8 @@ __host__ inline Level level_info(int N, int l) {
 // Index of the first element of a packed lower-triangular row r.
 __device__ __forceinline__ int tri(int r) { return (r * (r + 1)) >> 1; }
 
-__device__ __forceinline__ float rsqrt_t(float x) { return 1.0f / sqrtf(x); }
-__device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float rsqrt_t(float x) { return rsqrtf(x); }
+__device__ __forceinline__ double rsqrt_t(double x) { return rsqrt(x); }
 __device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
 __device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
 

@@ -277,12 +277,16 @@ hipError_t factor_level(T* L, T* E, const uint8_t* mask, int B, int N, int n, in
   const Level lv = level_info(N, lidx);
   if (lv.dim <= 0 || B <= 0) return hipSuccess;
 
+  kt_begin(kKcPotrf, st);
   hipLaunchKernelGGL(k_potrf<T>, dim3(lv.dim, B), dim3(kThreads), potrf_lds(n, sizeof(T)), st, L, mask, N, n, lv.s);
+  kt_end(kKcPotrf, st);
   if (lidx == Lv - 1) return hipGetLastError();
 
   const int nchunk = (n + kTrsmChunk - 1) / kTrsmChunk;
+  kt_begin(kKcTrsm, st);
   hipLaunchKernelGGL(k_trsm<T>, dim3(lv.dim, 2 * nchunk, B), dim3(kThreads), trsm_lds(n, sizeof(T)), st, L, E, mask, N,
                      n, nE, lv.s, lv.off, nchunk);
+  kt_end(kKcTrsm, st);
 
   const int ntile = (n + kTile - 1) / kTile;
   const int ntri = ntile * (ntile + 1) / 2;
@@ -292,9 +296,12 @@ hipError_t factor_level(T* L, T* E, const uint8_t* mask, int B, int N, int n, in
     if (lv.s - 1 + 2 * lv.s * t + lv.s < N) ++nF;
   // fill-in nodes are t = 1..nF (contiguous: i+s<N fails only at the tail)
   const int tasks = nT * ntri + nF * ntile * ntile;
-  if (tasks > 0)
+  if (tasks > 0) {
+    kt_begin(kKcSchur, st);
     hipLaunchKernelGGL(k_schur<T>, dim3(tasks, B), dim3(kThreads), 0, st, L, E, mask, N, n, nE, lv.s, lv.off,
                        lv.next_off, nT, ntile);
+    kt_end(kKcSchur, st);
+  }
   return hipGetLastError();
 }
 

@@ -264,8 +264,10 @@ hipError_t forward_step(const T* L, const T* E, T* x, const uint8_t* mask, int B
     items = N / (2 * s);
   }
   if (items <= 0 || B <= 0 || mch <= 0) return hipSuccess;
+  kt_begin(kKcForward, st);
   hipLaunchKernelGGL(k_forward_step<T>, dim3(items, mch, B), dim3(kThreads), lds, st, L, E, x, mask, N, n, m, nE, s,
                      off, step < 0 ? 1 : 0);
+  kt_end(kKcForward, st);
   return hipGetLastError();
 }
 
@@ -277,8 +279,10 @@ hipError_t backward_level(const T* L, const T* E, T* x, const uint8_t* mask, int
   const size_t lds = solve_lds_elems(n) * sizeof(T);
   const Level lv = level_info(N, lidx);
   if (lv.dim <= 0 || B <= 0 || mch <= 0) return hipSuccess;
+  kt_begin(kKcBackward, st);
   hipLaunchKernelGGL(k_backward_level<T>, dim3(lv.dim, mch, B), dim3(kThreads), lds, st, L, E, x, mask, N, n, m, nE,
                      lv.s, lv.off);
+  kt_end(kKcBackward, st);
   return hipGetLastError();
 }
 

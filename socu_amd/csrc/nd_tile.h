@@ -1,0 +1,165 @@
+// Wavefront-level building blocks for the blocked ND kernels on gfx950.
+//
+// * MFMA 16x16x4 wrappers for fp64 and fp32 with their lane layouts:
+//     A (16x4):  lane l holds A[l%16][l/16]
+//     B (4x16):  lane l holds B[l/16][l%16]
+//     D (16x16): fp32: lane l, reg r holds D[4*(l/16)+r][l%16]
+//                fp64: lane l, reg r holds D[4*r+(l/16)][l%16]
+// * wave_gemm: a 32x32 (2x2 MFMA tiles) accumulation C += A B with A, B
+//   read from LDS through element strides.
+// * tile_chol_inv: Cholesky and inverse of a 32x32 diagonal tile by one
+//   wavefront, row r in lane r's registers, column broadcasts through LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "nd_common.h"
+
+namespace socu {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  typedef f64x4 acc_t;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * r + (l >> 4); }
+  static __device__ __forceinline__ int col(int l, int r) { return l & 15; }
+};
+
+template <>
+struct Mfma<float> {
+  typedef f32x4 acc_t;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+  static __device__ __forceinline__ int col(int l, int r) { return l & 15; }
+};
+
+template <typename T>
+__device__ __forceinline__ typename Mfma<T>::acc_t acc_zero() {
+  typename Mfma<T>::acc_t z;
+  z[0] = z[1] = z[2] = z[3] = T(0);
+  return z;
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
+
+// Value of `v` in lane `src` (src must be wave-uniform).
+__device__ __forceinline__ double bcast_lane(double v, int src) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_readlane(lo, src);
+  hi = __builtin_amdgcn_readlane(hi, src);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float bcast_lane(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// C(32x32) += A(32xK) B(Kx32); A(i,k) = A0[i*ai + k*ak], B(k,j) = B0[k*bk + j*bj].
+// K must be a multiple of 4.  acc[ti][tj] is the 16x16 MFMA tile (ti, tj).
+template <typename T>
+__device__ __forceinline__ void wave_gemm(typename Mfma<T>::acc_t acc[2][2], const T* A0, int ai, int ak, const T* B0,
+                                          int bk, int bj, int K) {
+  const int l = lane_id();
+  const int i = l & 15, kq = l >> 4;
+  for (int k = 0; k < K; k += 4) {
+    const int kk = k + kq;
+    const T a0 = A0[i * ai + kk * ak];
+    const T a1 = A0[(i + 16) * ai + kk * ak];
+    const T b0 = B0[kk * bk + i * bj];
+    const T b1 = B0[kk * bk + (i + 16) * bj];
+    acc[0][0] = Mfma<T>::mma(a0, b0, acc[0][0]);
+    acc[0][1] = Mfma<T>::mma(a0, b1, acc[0][1]);
+    acc[1][0] = Mfma<T>::mma(a1, b0, acc[1][0]);
+    acc[1][1] = Mfma<T>::mma(a1, b1, acc[1][1]);
+  }
+}
+
+// One 16x16 tile: C += A(16xK) B(Kx16), same stride convention.
+template <typename T>
+__device__ __forceinline__ void wave_gemm16(typename Mfma<T>::acc_t& acc, const T* A0, int ai, int ak, const T* B0,
+                                            int bk, int bj, int K) {
+  const int l = lane_id();
+  const int i = l & 15, kq = l >> 4;
+  for (int k = 0; k < K; k += 4) {
+    const int kk = k + kq;
+    acc = Mfma<T>::mma(A0[i * ai + kk * ak], B0[kk * bk + i * bj], acc);
+  }
+}
+
+// Apply f(row, col, value) to every element of a 32x32 accumulator.
+template <typename T, typename F>
+__device__ __forceinline__ void acc_foreach(typename Mfma<T>::acc_t acc[2][2], F f) {
+  const int l = lane_id();
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) f(16 * ti + Mfma<T>::row(l, r), 16 * tj + Mfma<T>::col(l, r), acc[ti][tj][r]);
+}
+
+template <typename T, typename F>
+__device__ __forceinline__ void acc16_foreach(typename Mfma<T>::acc_t& acc, F f) {
+  const int l = lane_id();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) f(Mfma<T>::row(l, r), Mfma<T>::col(l, r), acc[r]);
+}
+
+// Cholesky and inverse of a 32x32 tile, executed by one whole wavefront.
+// S: tile in LDS, row-major with `pitch`; its lower triangle is the input.
+// On return the lower triangle holds L, the strict upper triangle holds the
+// transposed inverse (S[c][q] = Linv[q][c] for q > c) and dinv[r] = 1/L[r][r].
+//
+// Lanes 0-31 keep row r of the tile in u[]; lanes 32-63 keep column r of the
+// inverse (Gauss-Jordan: Linv = L_31^-1 ... L_0^-1).  Step k publishes column
+// k through the wave-private LDS buffer `col` (LDS operations of one
+// wavefront complete in issue order, so no barrier is needed); the pivot
+// comes from lane k by readlane, so the serial chain per step is one
+// readlane and one reciprocal square root.  Both halves then apply the same
+// update u[q] -= coef * col[q] with their own coefficient.
+template <typename T>
+__device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) {
+  const int lane = lane_id();
+  const int r = lane & 31;
+  const bool rowlane = lane < 32;
+  T u[32];
+  T rinv = T(1);
+#pragma unroll
+  for (int c = 0; c < 32; ++c) {
+    const T v = S[r * pitch + c];  // in-bounds; upper entries are discarded
+    u[c] = rowlane ? ((c <= r) ? v : T(0)) : ((c == r) ? T(1) : T(0));
+  }
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    if (rowlane) col[r] = u[k];
+    const T p = bcast_lane(u[k], k);
+    const T inv = rsqrt_t(p);
+    // row lanes: l = L[r][k] (0 for r <= k); inverse lanes: z_k = Linv[k][r]
+    const T lk = rowlane ? ((r > k) ? u[k] * inv : T(0)) : u[k] * inv;
+    const T coef = lk * inv;  // L[q][k] = col[q] * inv
+#pragma unroll
+    for (int q = k + 1; q < 32; ++q) u[q] -= coef * col[q];
+    u[k] = rowlane ? ((r == k) ? p * inv : ((r > k) ? lk : u[k])) : lk;
+    rinv = (r == k) ? inv : rinv;
+    // Materialise the registers every step: otherwise the optimiser sinks
+    // the updates to their last use and keeps every broadcast value alive.
+#pragma unroll
+    for (int q = 0; q < 32; ++q) asm volatile("" : "+v"(u[q]));
+  }
+#pragma unroll
+  for (int c = 0; c < 32; ++c)
+    if (rowlane ? (c <= r) : (c > r)) S[r * pitch + c] = u[c];
+  if (rowlane) dinv[r] = rinv;
+}
+
+}  // namespace socu

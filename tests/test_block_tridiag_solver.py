@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 TOLERANCE = {"float32": 1e-4, "float64": 1e-10}
 
 
-@functools.lru_cache(maxsize=8)
+@functools.lru_cache(maxsize=64)
 def prepare_test_data(n, N):
     A, L, E, b = nd.reference_test_problem(n, N)
     Lc = la.cholesky(A, lower=True)

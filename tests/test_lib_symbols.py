@@ -1,0 +1,68 @@
+"""The C-ABI library loads and exports every function include/*.h declares.
+
+No compute entry point is called (there is no GPU here); only the pure host
+functions are exercised.
+"""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from socu_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(socu_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    for must in ("socu_cholesky_factor", "socu_cholesky_solve", "socu_cholesky_factor_and_solve",
+                 "socu_cholesky_forward_substitution", "socu_cholesky_backward_substitution",
+                 "socu_calculate_off_diag_storage_len", "socu_plan_create", "socu_plan_execute",
+                 "socu_plan_destroy", "socu_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    for name in sorted(declared_functions()):
+        assert hasattr(lib, name), f"{name} declared in include/ but not exported"
+
+
+def test_ctypes_bindings_cover_the_header():
+    assert set(_lib.SIGNATURES) == declared_functions()
+
+
+def test_host_only_functions():
+    lib = _lib.load()
+    assert lib.socu_calculate_off_diag_storage_len(512) == 1013
+    assert lib.socu_calculate_recursive_iterations(512) == 10
+    assert lib.socu_max_block_size() >= 128
+    assert lib.socu_calculate_off_diag_storage_len(0) < 0
+    assert "horizon" in _lib.last_error()
+
+
+def test_invalid_arguments_fail_before_device_work():
+    lib = _lib.load()
+    rc = lib.socu_cholesky_factor(None, None, None, 1, 5, 4, 7, None)
+    assert rc == _lib.EINVAL and "dtype" in _lib.last_error()
+    rc = lib.socu_cholesky_factor(None, None, None, 1, 5, 4, _lib.DTYPE_F64, None)
+    assert rc == _lib.EINVAL and "NULL" in _lib.last_error()
+    rc = lib.socu_cholesky_factor(ctypes.c_void_p(8), ctypes.c_void_p(8), None, 1, 5, 1000, _lib.DTYPE_F64, None)
+    assert rc == _lib.EUNSUPPORTED
+    plan = ctypes.c_void_p()
+    rc = lib.socu_plan_create(ctypes.byref(plan), 9, None, None, None, None, 1, 5, 4, 1, _lib.DTYPE_F64, 0)
+    assert rc == _lib.EINVAL
+    # B = 0 is a valid no-op
+    assert lib.socu_cholesky_factor(None, None, None, 0, 5, 4, _lib.DTYPE_F64, None) == 0
