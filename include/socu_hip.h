@@ -86,13 +86,40 @@ int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const u
 int socu_plan_execute(socu_plan* plan, void* stream);
 int socu_plan_destroy(socu_plan* plan);
 
+/* Mixed-precision factor_and_solve with FP64 iterative refinement (no
+ * counterpart in the reference; an extension with the same data layout).
+ * L, E (FP64, the matrix Psi) are left untouched; x holds b on entry and the
+ * FP64 solution on exit.  The handle owns an FP32 working copy of (L, E) that
+ * is factored with FP32 MFMA Schur updates (precision SOCU_PREC_FP32: the
+ * diagonal blocks are factored and inverted in FP64, then rounded;
+ * SOCU_PREC_FP32_F32PIV: all in FP32), then runs up to max_iter correction
+ * solves d = Lp32^-T Lp32^-1 r, x += d, r = b - Psi x (FP64), stopping per
+ * system once ||r||_F <= tol ||b||_F.  The test runs on the device, so the
+ * whole solve is one hipGraph (use_graph) with no host synchronisation.
+ * socu_mixed_create allocates the workspace; execute allocates nothing.
+ * socu_mixed_stats synchronises `stream` and returns, per system, the last
+ * relative residual and the number of correction solves applied. */
+#define SOCU_PREC_FP32 0
+#define SOCU_PREC_FP32_F32PIV 1
+typedef struct socu_mixed socu_mixed;
+int socu_mixed_create(socu_mixed** out, const void* L, const void* E, void* x, const unsigned char* mask, int B, int N,
+                      int n, int m, int precision, double tol, int max_iter, int use_graph);
+int socu_mixed_execute(socu_mixed* handle, void* stream);
+int socu_mixed_stats(socu_mixed* handle, double* rel_resid, int* solves, void* stream);
+int socu_mixed_destroy(socu_mixed* handle);
+/* Runs the mixed solve once without the graph, with the per-kernel-class
+ * event timer of socu_profile_op (classes 0-4 as there, 5 = FP64 residual and
+ * refinement updates), and synchronises `stream`. */
+int socu_mixed_profile(socu_mixed* handle, void* stream, double* kernel_ms, int* kernel_launches);
+
 /* Instrumentation (no counterpart in the reference): runs `op` once like the
  * direct entry points, with a hipEvent pair around every kernel launch on
  * `stream`, synchronises `stream`, and returns per kernel class the summed
- * elapsed milliseconds and the launch count.  Classes: 0 potrf, 1 trsm,
- * 2 Schur update, 3 forward step, 4 backward level.  Creates events, so it is
+ * elapsed milliseconds and the launch count (arrays of SOCU_KERNEL_CLASSES).
+ * Classes: 0 potrf, 1 trsm, 2 Schur update, 3 forward step, 4 backward level,
+ * 5 residual / refinement.  Creates events, so it is
  * for measurement, not for hot loops. */
-#define SOCU_KERNEL_CLASSES 5
+#define SOCU_KERNEL_CLASSES 6
 int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
                     int dtype, void* stream, double* kernel_ms, int* kernel_launches);
 

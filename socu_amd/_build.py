@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsocu_hip.so")
 
-SOURCES = ["nd_factor.hip", "nd_solve.hip", "nd_blocked.hip", "capi.hip"]
+SOURCES = ["nd_blocked.hip", "nd_refine.hip", "capi.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

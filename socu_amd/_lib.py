@@ -29,7 +29,10 @@ OP_FACTOR_AND_SOLVE = 2
 OP_FORWARD = 3
 OP_BACKWARD = 4
 
-KERNEL_CLASSES = ("potrf", "trsm", "schur", "forward", "backward")
+KERNEL_CLASSES = ("potrf", "trsm", "schur", "forward", "backward", "refine")
+
+PREC_FP32 = 0
+PREC_FP32_F32PIV = 1
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -49,6 +52,11 @@ SIGNATURES = {
     "socu_plan_destroy": (_i, [_vp]),
     "socu_profile_op": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp,
                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
+    "socu_mixed_create": (_i, [ctypes.POINTER(_vp), _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, ctypes.c_double, _i, _i]),
+    "socu_mixed_execute": (_i, [_vp, _vp]),
+    "socu_mixed_stats": (_i, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), _vp]),
+    "socu_mixed_destroy": (_i, [_vp]),
+    "socu_mixed_profile": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
     "socu_last_error": (ctypes.c_char_p, []),
 }
 

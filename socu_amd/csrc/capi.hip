@@ -4,6 +4,7 @@
 // :1124-1228, :1294-1310, :1339-1369, :1435-1451, :1480-1512, :1586-1616,
 // :1666-1722); no Python runs per level.
 #include <stdarg.h>
+#include <math.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -54,7 +55,7 @@ template <typename T>
 int run_factor(T* L, T* E, const uint8_t* mask, int B, int N, int n, hipStream_t st) {
   const int Lv = socu::recursive_iterations(N);
   for (int l = 0; l < Lv; ++l) {
-    hipError_t e = socu::factor_level_blocked<T>(L, E, mask, B, N, n, l, st);
+    hipError_t e = socu::factor_level_blocked<T>(L, E, mask, B, N, n, l, false, st);
     if (e != hipSuccess) return hip_fail(e, "factor level launch");
   }
   return SOCU_OK;
@@ -112,9 +113,8 @@ int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N,
 int ensure_init() {
   static int state = 0;  // 0 = not yet, 1 = done
   if (state == 1) return SOCU_OK;
-  hipError_t e = socu::init_factor_kernels();
-  if (e == hipSuccess) e = socu::init_solve_kernels();
-  if (e == hipSuccess) e = socu::init_blocked_kernels();
+  hipError_t e = socu::init_blocked_kernels();
+  if (e == hipSuccess) e = socu::init_refine_kernels();
   if (e != hipSuccess) return hip_fail(e, "kernel set-up (hipFuncSetAttribute)");
   state = 1;
   return SOCU_OK;
@@ -307,6 +307,138 @@ int socu_plan_destroy(socu_plan* p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   delete p;
+  return SOCU_OK;
+}
+
+struct socu_mixed {
+  socu::MixedPlan p;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+int socu_mixed_create(socu_mixed** out, const void* L, const void* E, void* x, const unsigned char* mask, int B, int N,
+                      int n, int m, int precision, double tol, int max_iter, int use_graph) {
+  if (out == nullptr) return fail(SOCU_EINVAL, "output pointer is NULL");
+  *out = nullptr;
+  int rc = check_dims(L, E, x, true, B, N, n, m, SOCU_DTYPE_F64);
+  if (rc) return rc;
+  if (precision != SOCU_PREC_FP32 && precision != SOCU_PREC_FP32_F32PIV)
+    return fail(SOCU_EINVAL, "unknown factor precision %d", precision);
+  if (!(tol > 0.0)) return fail(SOCU_EINVAL, "tolerance must be positive");
+  if (max_iter < 1 || max_iter > 100) return fail(SOCU_EINVAL, "max_iter must be in [1, 100] (got %d)", max_iter);
+  if (socu::mixed_residual_lds(n, m) > 160 * 1024)
+    return fail(SOCU_EUNSUPPORTED, "n*m = %d too large for the residual kernel (max 5120)", n * m);
+  if (B > 0) {
+    rc = ensure_init();
+    if (rc) return rc;
+  }
+  socu_mixed* h = new socu_mixed();
+  h->p.L = static_cast<const double*>(L);
+  h->p.E = static_cast<const double*>(E);
+  h->p.x = static_cast<double*>(x);
+  h->p.mask = mask;
+  h->p.B = B;
+  h->p.N = N;
+  h->p.n = n;
+  h->p.m = m;
+  h->p.tol = tol;
+  h->p.max_iter = max_iter;
+  h->p.f64_pivots = precision == SOCU_PREC_FP32;
+  h->graph = nullptr;
+  h->exec = nullptr;
+  if (B == 0) {
+    *out = h;
+    return SOCU_OK;
+  }
+  hipError_t e = socu::mixed_alloc(h->p);
+  if (e != hipSuccess) {
+    socu::mixed_free(h->p);
+    delete h;
+    return hip_fail(e, "hipMalloc (mixed-precision workspace)");
+  }
+  if (use_graph) {
+    hipStream_t cs;
+    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      hipError_t e1 = socu::mixed_enqueue(h->p, cs);
+      hipGraph_t g = nullptr;
+      hipError_t e2 = hipStreamEndCapture(cs, &g);
+      e = e1 != hipSuccess ? e1 : e2;
+      if (e == hipSuccess) e = hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0);
+      h->graph = g;
+    }
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) {
+      if (h->exec) (void)hipGraphExecDestroy(h->exec);
+      if (h->graph) (void)hipGraphDestroy(h->graph);
+      socu::mixed_free(h->p);
+      delete h;
+      return hip_fail(e, "graph capture (mixed-precision solve)");
+    }
+  }
+  *out = h;
+  return SOCU_OK;
+}
+
+int socu_mixed_execute(socu_mixed* h, void* stream) {
+  if (h == nullptr) return fail(SOCU_EINVAL, "handle is NULL");
+  if (h->p.B == 0) return SOCU_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipError_t e = h->exec ? hipGraphLaunch(h->exec, st) : socu::mixed_enqueue(h->p, st);
+  if (e != hipSuccess) return hip_fail(e, "mixed-precision solve launch");
+  return SOCU_OK;
+}
+
+int socu_mixed_stats(socu_mixed* h, double* rel_resid, int* solves, void* stream) {
+  if (h == nullptr || rel_resid == nullptr || solves == nullptr) return fail(SOCU_EINVAL, "NULL argument");
+  const int B = h->p.B;
+  if (B == 0) return SOCU_OK;
+  std::vector<double> st(4 * (size_t)B);
+  hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+  if (e == hipSuccess) e = hipMemcpy(st.data(), h->p.state, sizeof(double) * 4 * B, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "reading refinement state");
+  for (int b = 0; b < B; ++b) {
+    const double bb = st[4 * b], rr = st[4 * b + 1];
+    rel_resid[b] = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+    solves[b] = (int)st[4 * b + 3];
+  }
+  return SOCU_OK;
+}
+
+int socu_mixed_profile(socu_mixed* h, void* stream, double* kernel_ms, int* kernel_launches) {
+  if (h == nullptr || kernel_ms == nullptr || kernel_launches == nullptr) return fail(SOCU_EINVAL, "NULL argument");
+  for (int c = 0; c < socu::kKcCount; ++c) {
+    kernel_ms[c] = 0.0;
+    kernel_launches[c] = 0;
+  }
+  if (h->p.B == 0) return SOCU_OK;
+  g_timer = TimerState();
+  g_timer.active = true;
+  hipError_t e = socu::mixed_enqueue(h->p, static_cast<hipStream_t>(stream));
+  g_timer.active = false;
+  hipError_t e2 = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+  for (size_t q = 0; q < g_timer.cls.size(); ++q) {
+    float ms = 0.f;
+    if (e == hipSuccess && e2 == hipSuccess &&
+        hipEventElapsedTime(&ms, g_timer.ev[2 * q], g_timer.ev[2 * q + 1]) == hipSuccess) {
+      kernel_ms[g_timer.cls[q]] += ms;
+      kernel_launches[g_timer.cls[q]] += 1;
+    }
+  }
+  for (hipEvent_t ev : g_timer.ev) (void)hipEventDestroy(ev);
+  g_timer = TimerState();
+  if (e != hipSuccess) return hip_fail(e, "mixed-precision solve launch (profile)");
+  if (e2 != hipSuccess) return hip_fail(e2, "hipStreamSynchronize (profile)");
+  return SOCU_OK;
+}
+
+int socu_mixed_destroy(socu_mixed* h) {
+  if (h == nullptr) return SOCU_OK;
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  socu::mixed_free(h->p);
+  delete h;
   return SOCU_OK;
 }
 

@@ -1,23 +1,26 @@
 // Blocked ND kernels for block size n <= 128 on gfx950: blocks are padded to
 // NP = 32*NB (NB = ceil(n/32)) inside LDS/registers only; HBM accesses stay
-// within the n x n blocks.  One kernel launch per phase per level; no atomics.
+// within the n x n blocks.  One or two kernel launches per phase per level; no
+// atomics, so results are bitwise reproducible.
 //
-//   k_potrf_blk  one workgroup per node:   L_i <- chol(L_i), blocked
+//   k_potrf_blk  one workgroup per node:  L_i <- chol(L_i), blocked
 //                right-looking over 32x32 tiles (reference: potrf_blocked,
 //                socu/blas/potrf.py:97-119).  Each diagonal tile is factored
-//                and inverted by one wavefront (tile_chol_inv); the panel and
-//                trailing updates run on MFMA.  The transposed tile inverses
-//                are left in the strict upper triangle of the diagonal tiles
-//                (the upper triangle of L is not part of the factor).
+//                and inverted by one wavefront (tile_chol_inv); panel and
+//                trailing updates run on MFMA.  Then the full inverse Linv_i is
+//                formed (MFMA) and stored transposed in the strict upper
+//                triangle of L_i, which is not part of the factor (the
+//                reference only ever reads tril(L)).
 //   k_trsm_blk   one workgroup per (node, 32-wide chunk of R rows / Lc cols):
-//                R <- R L^{-T}, Lc <- L^{-1} Lc by blocked substitution with
-//                the tile inverses (trsm.py:94-120, :202-228).
+//                R <- R Linv^T, Lc <- Linv Lc as MFMA products with the stored
+//                inverse (replaces trsm_rltn / trsm_llnn, trsm.py:94-120,
+//                :202-228).
 //   k_schur_blk  one workgroup per 64x64 output tile, MFMA, owner computes:
 //                L_j -= R R^T + Lc^T Lc,  F -= R Lc (syrk.py, gemm.py).
-//   k_fwd_blk / k_bwd_blk  forward step / backward level (see nd_solve.hip
-//                for the step structure): coalesced block-times-panel
-//                products and blocked triangular solves with one-wave tile
-//                solves.
+//   k_sweep_update, k_apply_fwd, k_apply_bwd  forward step / backward level:
+//                row-split block-times-panel updates, then triangular GEMVs
+//                with the stored inverses (replaces K9-K16,
+//                block_tridiag_solver.py:425-736).
 #include "nd_common.h"
 #include "nd_internal.h"
 #include "nd_tile.h"
@@ -30,8 +33,10 @@ namespace socu {
 // NF != 0 instantiates the kernels for a fixed block size n == NF (a multiple
 // of 32), so that all index arithmetic is compile-time; NF == 0 is the
 // general (padded) version.
-template <typename T, int NB, int NF>
-__global__ __launch_bounds__(kThreads) void k_potrf_blk(T* __restrict__ L, const uint8_t* __restrict__ mask, int N,
+// TS is the storage type of L; T the compute type (TS = float, T = double
+// factors FP32 blocks with FP64 pivots and updates).
+template <typename TS, int NB, int NF, typename T = TS>
+__global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, const uint8_t* __restrict__ mask, int N,
                                                        int n_, int s) {
   const int n = NF ? NF : n_;
   constexpr int NP = 32 * NB;
@@ -39,11 +44,11 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(T* __restrict__ L, const
   const int b = blockIdx.y;
   if (masked_out(mask, b)) return;
   const int i = s - 1 + 2 * s * blockIdx.x;
-  T* A = L + ((size_t)b * N + i) * (size_t)n * n;
+  TS* A = L + ((size_t)b * N + i) * (size_t)n * n;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* S = reinterpret_cast<T*>(smem_raw);
-  T* dinv = S + NP * P;
-  T* col = dinv + 32;
+  T* dinv = S + NP * P;  // 1/L[r][r] for every row
+  T* col = dinv + NP;
   const int tid = threadIdx.x, wave = wave_id(), l = lane_id();
 
   // The block is read as one contiguous n*n run (coalesced, unconditional
@@ -54,7 +59,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(T* __restrict__ L, const
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     const int idx = tid + e * kThreads;
-    v[e] = A[min(idx, nn - 1)];
+    v[e] = (T)A[min(idx, nn - 1)];
   }
   for (int idx = tid; idx < NP * NP; idx += kThreads) {
     const int r = idx / NP, c = idx - r * NP;
@@ -71,7 +76,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(T* __restrict__ L, const
 
   for (int kb = 0; kb < NB; ++kb) {
     T* D = S + 32 * kb * P + 32 * kb;
-    if (wave == 0) tile_chol_inv<T>(D, P, dinv, col);
+    if (wave == 0) tile_chol_inv<T>(D, P, dinv + 32 * kb, col);
     __syncthreads();
     // panel: tile (ti, kb) <- tile (ti, kb) Linv^T
     for (int ti = kb + 1 + wave; ti < NB; ti += kThreads / 64) {
@@ -84,8 +89,8 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(T* __restrict__ L, const
         const T a0 = X[li * P + kk], a1 = X[(li + 16) * P + kk];
         // B(k, c) = Linv[c][k]: strict upper of D holds it for c > k
         const int c0 = li, c1 = li + 16;
-        const T b0 = (c0 > kk) ? D[kk * P + c0] : ((c0 == kk) ? dinv[kk] : T(0));
-        const T b1 = (c1 > kk) ? D[kk * P + c1] : ((c1 == kk) ? dinv[kk] : T(0));
+        const T b0 = (c0 > kk) ? D[kk * P + c0] : ((c0 == kk) ? dinv[32 * kb + kk] : T(0));
+        const T b1 = (c1 > kk) ? D[kk * P + c1] : ((c1 == kk) ? dinv[32 * kb + kk] : T(0));
         acc[0][0] = Mfma<T>::mma(a0, b0, acc[0][0]);
         acc[0][1] = Mfma<T>::mma(a0, b1, acc[0][1]);
         acc[1][0] = Mfma<T>::mma(a1, b0, acc[1][0]);
@@ -110,17 +115,58 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(T* __restrict__ L, const
     __syncthreads();
   }
 
+  // Off-diagonal tiles of the inverse, column by column:
+  //   Linv_ij = -Dinv_i sum_{k=j}^{i-1} L_ik Linv_kj   (j < i),  Linv_jj = Dinv_j.
+  // Wave w owns columns j = w, w+4, ...; Linv_ij is kept transposed in the
+  // upper tile (j, i): S[32j+c][32i+r] = Linv_ij[r][c].  The partial sum is
+  // parked there too, then overwritten in place by the same wave.
+  for (int j = wave; j < NB - 1; j += kThreads / 64) {
+    for (int i = j + 1; i < NB; ++i) {
+      typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+      for (int k = j; k < i; ++k) {
+        const T* Lik = S + 32 * i * P + 32 * k;
+        if (k == j) {
+          const T* Dj = S + 32 * j * P + 32 * j;
+          wave_gemm_f<T>(
+              acc, [&](int r, int kk) { return Lik[r * P + kk]; },
+              [&](int kk, int c) {
+                return (kk > c) ? Dj[c * P + kk] : ((kk == c) ? dinv[32 * j + c] : T(0));
+              },
+              32);
+        } else {
+          const T* Ukj = S + 32 * j * P + 32 * k;  // holds Linv_kj transposed
+          wave_gemm<T>(acc, Lik, P, 1, Ukj, 1, P, 32);
+        }
+      }
+      T* Uji = S + 32 * j * P + 32 * i;
+      acc_foreach<T>(acc, [&](int r, int c, T v) { Uji[c * P + r] = v; });
+      typename Mfma<T>::acc_t inv[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+      const T* Di = S + 32 * i * P + 32 * i;
+      wave_gemm_f<T>(
+          inv, [&](int r, int kk) { return (r > kk) ? Di[kk * P + r] : ((r == kk) ? dinv[32 * i + r] : T(0)); },
+          [&](int kk, int c) { return Uji[c * P + kk]; }, 32);
+      acc_foreach<T>(inv, [&](int r, int c, T v) { Uji[c * P + r] = -v; });
+    }
+  }
+  __syncthreads();
+
   // Whole block written back as one contiguous run: tril(L) is the factor;
-  // the strict upper triangle (not part of the factor) carries the diagonal
-  // tiles' transposed inverses and stale values elsewhere.
+  // the strict upper triangle, which is not part of the factor, carries the
+  // transposed inverse of the diagonal block (U[k][r] = Linv[r][k], k < r)
+  // for the trsm and solve kernels.
   for (int idx = tid; idx < nn; idx += kThreads) {
     const int r = idx / n, c = idx - r * n;
-    A[idx] = S[r * P + c];
+    A[idx] = (TS)S[r * P + c];
   }
 }
 
 // ---------------------------------------------------------------------------
-// trsm
+// trsm with the explicit inverse left by k_potrf_blk in the strict upper
+// triangle of L_i (U[k][r] = Linv[r][k] for k < r, 1/L[r][r] on the diagonal):
+//   R task  (32 rows of R):   R  <- R Linv^T,   wave q: output columns 32q..
+//   Lc task (32 cols of Lc):  Lc <- Linv Lc,    wave q: output rows 32q..
+// Both are MFMA products with the upper tiles (kb, q), kb <= q, of U; no
+// sequential substitution remains.
 // ---------------------------------------------------------------------------
 template <typename T, int NB, int NF>
 __global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, T* __restrict__ E,
@@ -143,109 +189,80 @@ __global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, 
   const T* A = L + ((size_t)b * N + i) * (size_t)n * n;
   T* X = E + ((size_t)b * nE + off + (isR ? i / s : (i - s) / s)) * (size_t)n * n;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* Lt = reinterpret_cast<T*>(smem_raw);  // NT tiles of 32 x TP
-  T* dinv = Lt + NT * 32 * TP;             // NP
-  T* Z = dinv + NP;                        // NP x TP
-  const int tid = threadIdx.x, wave = wave_id(), l = lane_id();
+  T* Ut = reinterpret_cast<T*>(smem_raw);  // NT upper tiles (kb <= q) of U, 32 x TP each
+  T* Xs = Ut + NT * 32 * TP;                // R task: 32 x (NP+1); Lc task: NP x TP
+  constexpr int XP = NP + 1;
+  const int tid = threadIdx.x, wave = wave_id();
 
-  // Issue every load (tiles of L_i, pivots, the panel) before storing any.
-  constexpr int PT = 32 * 32 / kThreads;  // elements per thread per tile
-  T lv[NT][PT];
+  // Issue every load before storing any: upper tiles of U (diagonal entries
+  // turned into reciprocals, lower parts of diagonal tiles zeroed) and the
+  // 32-wide chunk of R or Lc.
+  constexpr int PT = 32 * 32 / kThreads;
+  T uv[NT][PT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    int ti = 0, q = t;
-    while (q > ti) { q -= ti + 1; ++ti; }
-    const int tj = q;
+    int q = 0, kb = t;
+    while (kb > q) { kb -= q + 1; ++q; }  // t = q(q+1)/2 + kb, kb <= q
 #pragma unroll
     for (int e = 0; e < PT; ++e) {
       const int idx = tid + e * kThreads;
-      const int R = 32 * ti + (idx >> 5), C = 32 * tj + (idx & 31);
+      const int R = 32 * kb + (idx >> 5), C = 32 * q + (idx & 31);
       const T g = A[(size_t)min(R, n - 1) * n + min(C, n - 1)];
-      lv[t][e] = (R < n && C < n) ? g : ((R == C) ? T(1) : T(0));
+      uv[t][e] = (R < n && C < n) ? ((R < C) ? g : ((R == C) ? T(1) / g : T(0))) : ((R == C) ? T(1) : T(0));
     }
   }
   constexpr int PZ = 32 * NP / kThreads;
   T zv[PZ];
-  if (isR) {
 #pragma unroll
-    for (int e = 0; e < PZ; ++e) {
-      const int idx = tid + e * kThreads;
-      const int c = idx / NP, k = idx - c * NP;
-      const T g = X[(size_t)(c0 + min(c, w - 1)) * n + min(k, n - 1)];
-      zv[e] = (c < w && k < n) ? g : T(0);
-    }
-  } else {
-#pragma unroll
-    for (int e = 0; e < PZ; ++e) {
-      const int idx = tid + e * kThreads;
-      const int k = idx >> 5, c = idx & 31;
+  for (int e = 0; e < PZ; ++e) {
+    const int idx = tid + e * kThreads;
+    if (isR) {
+      const int r = idx / NP, k = idx - r * NP;  // R[c0+r][k], row reads
+      const T g = X[(size_t)(c0 + min(r, w - 1)) * n + min(k, n - 1)];
+      zv[e] = (r < w && k < n) ? g : T(0);
+    } else {
+      const int k = idx >> 5, c = idx & 31;  // Lc[k][c0+c]
       const T g = X[(size_t)min(k, n - 1) * n + c0 + min(c, w - 1)];
       zv[e] = (c < w && k < n) ? g : T(0);
     }
   }
-  for (int k = tid; k < NP; k += kThreads) dinv[k] = (k < n) ? T(1) / A[(size_t)k * n + k] : T(1);
 #pragma unroll
   for (int t = 0; t < NT; ++t)
 #pragma unroll
     for (int e = 0; e < PT; ++e) {
       const int idx = tid + e * kThreads;
-      Lt[t * 32 * TP + (idx >> 5) * TP + (idx & 31)] = lv[t][e];
+      Ut[t * 32 * TP + (idx >> 5) * TP + (idx & 31)] = uv[t][e];
     }
-  if (isR) {
 #pragma unroll
-    for (int e = 0; e < PZ; ++e) {
-      const int idx = tid + e * kThreads;
-      const int c = idx / NP, k = idx - c * NP;
-      Z[k * TP + c] = zv[e];
-    }
-  } else {
-#pragma unroll
-    for (int e = 0; e < PZ; ++e) {
-      const int idx = tid + e * kThreads;
-      Z[(idx >> 5) * TP + (idx & 31)] = zv[e];
-    }
+  for (int e = 0; e < PZ; ++e) {
+    const int idx = tid + e * kThreads;
+    if (isR) Xs[(idx / NP) * XP + (idx % NP)] = zv[e];
+    else Xs[(idx >> 5) * TP + (idx & 31)] = zv[e];
   }
   __syncthreads();
 
-  const int qi = wave >> 1, qj = wave & 1;
-  const int li = l & 15, kq = l >> 4;
-  for (int kb = 0; kb < NB; ++kb) {
-    if (kb > 0) {
-      typename Mfma<T>::acc_t acc = acc_zero<T>();
-      for (int j = 0; j < kb; ++j) {
-        const T* At = Lt + (kb * (kb + 1) / 2 + j) * 32 * TP + 16 * qi * TP;
-        wave_gemm16<T>(acc, At, TP, 1, Z + 32 * j * TP + 16 * qj, TP, 1, 32);
+  for (int q = wave; q < NB; q += kThreads / 64) {
+    typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+    for (int kb = 0; kb <= q; ++kb) {
+      const T* Utile = Ut + (q * (q + 1) / 2 + kb) * 32 * TP;  // U[32kb.., 32q..]
+      if (isR) {
+        // out[r][c] += sum_k R[r][32kb+k] U[32kb+k][32q+c]
+        wave_gemm<T>(acc, Xs + 32 * kb, XP, 1, Utile, TP, 1, 32);
+      } else {
+        // out[r][c] += sum_k Linv[32q+r][32kb+k] Lc[32kb+k][c],  Linv[32q+r][32kb+k] = U[32kb+k][32q+r]
+        wave_gemm<T>(acc, Utile, 1, TP, Xs + 32 * kb * TP, TP, 1, 32);
       }
-      T* Zq = Z + (32 * kb + 16 * qi) * TP + 16 * qj;
-      acc16_foreach<T>(acc, [&](int r, int c, T v) { Zq[r * TP + c] -= v; });
-      __syncthreads();
     }
-    // Z[kb] <- Linv_kk Z[kb]; A(r,k) = Linv[r][k] = D[k][r] (r > k), dinv (r == k)
-    const T* D = Lt + (kb * (kb + 1) / 2 + kb) * 32 * TP;
-    typename Mfma<T>::acc_t acc = acc_zero<T>();
-#pragma unroll
-    for (int k = 0; k < 32; k += 4) {
-      const int kk = k + kq;
-      const int r = 16 * qi + li;
-      const T a = (r > kk) ? D[kk * TP + r] : ((r == kk) ? dinv[32 * kb + kk] : T(0));
-      const T bv = Z[(32 * kb + kk) * TP + 16 * qj + li];
-      acc = Mfma<T>::mma(a, bv, acc);
-    }
-    __syncthreads();
-    T* Zq = Z + (32 * kb + 16 * qi) * TP + 16 * qj;
-    acc16_foreach<T>(acc, [&](int r, int c, T v) { Zq[r * TP + c] = v; });
-    __syncthreads();
-  }
-
-  if (isR) {
-    for (int idx = tid; idx < w * n; idx += kThreads) {
-      const int c = idx / n, k = idx - c * n;
-      X[(size_t)(c0 + c) * n + k] = Z[k * TP + c];
-    }
-  } else {
-    for (int idx = tid; idx < n * w; idx += kThreads) {
-      const int k = idx / w, c = idx - k * w;
-      X[(size_t)k * n + c0 + c] = Z[k * TP + c];
+    if (isR) {
+      acc_foreach<T>(acc, [&](int r, int c, T v) {
+        const int C = 32 * q + c;
+        if (r < w && C < n) X[(size_t)(c0 + r) * n + C] = v;
+      });
+    } else {
+      acc_foreach<T>(acc, [&](int r, int c, T v) {
+        const int R = 32 * q + r;
+        if (c < w && R < n) X[(size_t)R * n + c0 + c] = v;
+      });
     }
   }
 }
@@ -394,265 +411,227 @@ __global__ __launch_bounds__(kThreads) void k_schur_blk(T* __restrict__ L, T* __
 }
 
 // ---------------------------------------------------------------------------
-// solves
+// solves, with the explicit inverses of the diagonal blocks (see k_potrf_blk)
+//
+// Forward step l (levels ascending):
+//   k_sweep_update<fwd>: for every receiving block j = 2s(u+1)-1 and every
+//       32-row block rb:  y_j[rb] -= R_{j-s}[rb,:] y_{j-s} + Lc_{j+s}[:,rb]^T y_{j+s}
+//   k_apply_fwd:         for every node j of level l+1:  y_j <- Linv_j y_j
+// Backward level l (levels descending):
+//   k_sweep_update<bwd>: for every node i and 32-row block rb:
+//                         x_i[rb] -= R_i[:,rb]^T x_{i+s} + Lc_i[rb,:] x_{i-s}
+//   k_apply_bwd:         x_i <- Linv_i^T x_i
+// The updates spread each block over NB workgroups (one per 32 rows) so that
+// the E blocks stream through many CUs; the applies are triangular GEMVs.
 // ---------------------------------------------------------------------------
-// Y (n x w panel in LDS, pitch YP) -= A Y1 with A an n x n row-major block
-// in HBM: 16 lanes per row, each holding a contiguous slice of the row, then a
-// 16-lane shuffle reduction per right-hand side.  A wave covers 4 rows per
-// pass; every pass's loads are issued before the first is used.
-template <typename T, int NB>
-__device__ __forceinline__ void panel_sub_AY(T* Y, const T* __restrict__ A, const T* Y1, int n, int w, int YP) {
-  constexpr int NP = 32 * NB;
-  constexpr int KPL = NP / 16;                     // elements per lane
-  constexpr int PASSES = NP / (4 * (kThreads / 64));  // 4 rows per wave per pass
-  const int l = lane_id(), wave = wave_id();
-  const int sub = l & 15, rsub = l >> 4;
-  T a[PASSES][KPL];
-#pragma unroll
-  for (int p = 0; p < PASSES; ++p) {
-    const int r = 4 * (wave + p * (kThreads / 64)) + rsub;
-#pragma unroll
-    for (int e = 0; e < KPL; ++e) {
-      const int k = sub * KPL + e;
-      const T g = A[(size_t)min(r, n - 1) * n + min(k, n - 1)];
-      a[p][e] = (r < n && k < n) ? g : T(0);
-    }
-  }
-#pragma unroll
-  for (int p = 0; p < PASSES; ++p) {
-    const int r = 4 * (wave + p * (kThreads / 64)) + rsub;
-    for (int c = 0; c < w; ++c) {
-      T acc = T(0);
-#pragma unroll
-      for (int e = 0; e < KPL; ++e) acc += a[p][e] * Y1[(sub * KPL + e) * YP + c];
-#pragma unroll
-      for (int o = 8; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 16);
-      if (sub == 0 && r < n) Y[r * YP + c] -= acc;
-    }
-  }
-}
+constexpr int kRP = kRhsChunk + 1;  // pitch of right-hand-side panels in LDS
 
-// Y -= A^T Y1: thread (r, part) holds A[k][r] for its K part (reads coalesced
-// in r); the parts subtract in a fixed order, one barrier apart.
-template <typename T, int NB>
-__device__ __forceinline__ void panel_sub_AtY(T* Y, const T* __restrict__ A, const T* Y1, int n, int w, int YP) {
+// One 32-row block of the update.  "Row operand" A (rows r0..r0+31 of an
+// n x n block, times Ya) and "column operand" C (columns r0..r0+31, i.e.
+// C[:, rb]^T, times Yc); either may be absent.
+template <typename T, int NB, int NF, bool BWD>
+__global__ __launch_bounds__(kThreads) void k_sweep_update(const T* __restrict__ E, T* __restrict__ x,
+                                                          const uint8_t* __restrict__ mask, int N, int n_, int m,
+                                                          int nE, int s, int off) {
+  const int n = NF ? NF : n_;
   constexpr int NP = 32 * NB;
-  constexpr int PARTS = kThreads / NP;
-  constexpr int KP = (NP + PARTS - 1) / PARTS;
-  const int t = threadIdx.x;
-  const int r = t % NP, part = t / NP;
-  const bool act = part < PARTS && r < n;
-  T a[KP];
-#pragma unroll
-  for (int e = 0; e < KP; ++e) {
-    const int k = part * KP + e;
-    const T g = A[(size_t)min(k, n - 1) * n + min(r, n - 1)];
-    a[e] = (act && k < n) ? g : T(0);
+  constexpr int AP = NP + 1;
+  const int b = blockIdx.z;
+  if (masked_out(mask, b)) return;
+  const int rb = blockIdx.y % NB, ch = blockIdx.y / NB;
+  const int r0 = 32 * rb;
+  if (r0 >= n) return;
+  const int c0 = ch * kRhsChunk;
+  const int w = min(kRhsChunk, m - c0);
+  const size_t nn = (size_t)n * n, nm = (size_t)n * m;
+  const T* Eb = E + (size_t)b * nE * nn;
+  T* xb = x + (size_t)b * N * nm;
+
+  int j, ja = -1, jc = -1;
+  const T* Aop = nullptr;  // rows r0.. used as A[r][k]
+  const T* Cop = nullptr;  // columns r0.. used as C[k][r]
+  if (!BWD) {
+    j = 2 * s * (blockIdx.x + 1) - 1;
+    ja = j - s;
+    Aop = Eb + (size_t)(off + (j - s) / s) * nn;  // R of node j-s: block (j, j-s)
+    if (j + s < N) {
+      jc = j + s;
+      Cop = Eb + (size_t)(off + j / s) * nn;  // Lc of node j+s: block (j+s, j)
+    }
+  } else {
+    j = s - 1 + 2 * s * blockIdx.x;
+    if (j >= s) {
+      ja = j - s;
+      Aop = Eb + (size_t)(off + (j - s) / s) * nn;  // Lc of node j: block (j, j-s)
+    }
+    if (j + s < N) {
+      jc = j + s;
+      Cop = Eb + (size_t)(off + j / s) * nn;  // R of node j: block (j+s, j)
+    }
+    if (ja < 0 && jc < 0) return;
   }
-  for (int c = 0; c < w; ++c) {
+
+  __shared__ T Ya[NP * kRP];
+  __shared__ T Yc[NP * kRP];
+  __shared__ T As[32 * AP];
+  __shared__ T Cs[NP * 33];
+  const int tid = threadIdx.x;
+  constexpr int PA = 32 * NP / kThreads;  // elements per thread of a 32 x NP slab
+  T av[PA], cv[PA];
+#pragma unroll
+  for (int e = 0; e < PA; ++e) {
+    const int idx = tid + e * kThreads;
+    const int ra = idx / NP, ka = idx - ra * NP;  // A[r0+ra][ka]
+    const int kc = idx >> 5, rc = idx & 31;       // C[kc][r0+rc]
+    T ga = T(0), gc = T(0);
+    if (Aop) ga = Aop[(size_t)min(r0 + ra, n - 1) * n + min(ka, n - 1)];
+    if (Cop) gc = Cop[(size_t)min(kc, n - 1) * n + min(r0 + rc, n - 1)];
+    av[e] = (Aop && r0 + ra < n && ka < n) ? ga : T(0);
+    cv[e] = (Cop && kc < n && r0 + rc < n) ? gc : T(0);
+  }
+  for (int q = tid; q < NP * w; q += kThreads) {
+    const int k = q / w, c = q - k * w;
+    Ya[k * kRP + c] = (ja >= 0 && k < n) ? xb[(size_t)ja * nm + (size_t)k * m + c0 + c] : T(0);
+    Yc[k * kRP + c] = (jc >= 0 && k < n) ? xb[(size_t)jc * nm + (size_t)k * m + c0 + c] : T(0);
+  }
+#pragma unroll
+  for (int e = 0; e < PA; ++e) {
+    const int idx = tid + e * kThreads;
+    As[(idx / NP) * AP + (idx % NP)] = av[e];
+    Cs[(idx >> 5) * 33 + (idx & 31)] = cv[e];
+  }
+  __syncthreads();
+
+  // 8 lanes per output (row r, column c), each summing NP/8 of K, then a
+  // width-8 shuffle reduction.
+  constexpr int KP = NP / 8;
+  const int part = tid & 7;
+  for (int q = tid >> 3; q < 32 * w; q += kThreads / 8) {
+    const int r = q & 31, c = q >> 5;
     T acc = T(0);
 #pragma unroll
     for (int e = 0; e < KP; ++e) {
       const int k = part * KP + e;
-      if (k < NP) acc += a[e] * Y1[k * YP + c];
+      acc += As[r * AP + k] * Ya[k * kRP + c] + Cs[k * 33 + r] * Yc[k * kRP + c];
     }
-    for (int p = 0; p < PARTS; ++p) {
-      if (act && part == p) Y[r * YP + c] -= acc;
-      __syncthreads();
+#pragma unroll
+    for (int o = 4; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 8);
+    if (part == 0 && r0 + r < n) {
+      T* dst = xb + (size_t)j * nm + (size_t)(r0 + r) * m + c0 + c;
+      *dst -= acc;
     }
   }
 }
 
-// Y <- L^{-1} Y (forward, lower) using tril(L) from HBM; blocked over 32-row
-// tiles, the diagonal tile solved by wave 0 with register broadcasts.
-template <typename T, int NB>
-__device__ __forceinline__ void panel_lower_solve(T* Y, const T* __restrict__ Lb, int n, int w, int YP) {
-  const int l = lane_id(), wave = wave_id();
-  for (int kb = 0; kb < NB; ++kb) {
-    const int base = 32 * kb;
-    if (wave == 0) {
-      const int r = l & 31, R = base + r;
-      T a[32];
-      T rdiag = T(1);
-#pragma unroll
-      for (int c = 0; c < 32; ++c) {
-        const int C = base + c;
-        const T g = Lb[(size_t)min(R, n - 1) * n + min(C, n - 1)];
-        a[c] = (R < n && C < n) ? ((c <= r) ? g : T(0)) : ((R == C) ? T(1) : T(0));
-        if (c == r) rdiag = T(1) / a[c];
-      }
-      for (int c = 0; c < w; ++c) {
-        T y = (R < n) ? Y[R * YP + c] : T(0);
-#pragma unroll
-        for (int j = 0; j < 32; ++j) {
-          const T yj = bcast_lane(y, j) * bcast_lane(rdiag, j);
-          y = (r == j) ? yj : ((r > j) ? y - a[j] * yj : y);
-        }
-        if (l < 32 && R < n) Y[R * YP + c] = y;
-      }
-    }
-    __syncthreads();
-    // rows below the tile: Y[r] -= L[r][base:base+32] Y[base:base+32]
-    const int rows = n - base - 32;
-    if (rows > 0) {
-      for (int q = threadIdx.x; q < rows * w; q += kThreads) {
-        const int c = q / rows, r = base + 32 + (q - c * rows);
-        T lrow[32];
-#pragma unroll
-        for (int k = 0; k < 32; ++k) {
-          const T g = Lb[(size_t)r * n + min(base + k, n - 1)];
-          lrow[k] = (base + k < n) ? g : T(0);
-        }
-        T acc = T(0);
-#pragma unroll
-        for (int k = 0; k < 32; ++k) acc += lrow[k] * Y[(base + k) * YP + c];
-        Y[r * YP + c] -= acc;
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Y <- L^{-T} Y (backward) using tril(L) from HBM.
-template <typename T, int NB>
-__device__ __forceinline__ void panel_upper_solve_T(T* Y, const T* __restrict__ Lb, int n, int w, int YP) {
-  const int l = lane_id(), wave = wave_id();
-  for (int kb = NB - 1; kb >= 0; --kb) {
-    const int base = 32 * kb;
-    if (wave == 0) {
-      // lane q holds column q of the diagonal tile: bcol[j] = L[base+j][base+q]
-      const int q = l & 31, Q = base + q;
-      T bcol[32];
-      T rdiag = T(1);
-#pragma unroll
-      for (int j = 0; j < 32; ++j) {
-        const int R = base + j;
-        const T g = Lb[(size_t)min(R, n - 1) * n + min(Q, n - 1)];
-        bcol[j] = (R < n && Q < n) ? ((j >= q) ? g : T(0)) : ((R == Q) ? T(1) : T(0));
-        if (j == q) rdiag = T(1) / bcol[j];
-      }
-      for (int c = 0; c < w; ++c) {
-        T y = (Q < n) ? Y[Q * YP + c] : T(0);
-#pragma unroll
-        for (int j = 31; j >= 0; --j) {
-          const T yj = bcast_lane(y, j) * bcast_lane(rdiag, j);
-          y = (q == j) ? yj : ((q < j) ? y - bcol[j] * yj : y);
-        }
-        if (l < 32 && Q < n) Y[Q * YP + c] = y;
-      }
-    }
-    __syncthreads();
-    // rows above the tile: Y[r] -= L[base:base+32][r]^T Y[base:base+32]
-    if (base > 0) {
-      for (int qq = threadIdx.x; qq < base * w; qq += kThreads) {
-        const int c = qq / base, r = qq - c * base;
-        T lcol[32];
-#pragma unroll
-        for (int k = 0; k < 32; ++k) {
-          const T g = Lb[(size_t)min(base + k, n - 1) * n + r];
-          lcol[k] = (base + k < n) ? g : T(0);
-        }
-        T acc = T(0);
-#pragma unroll
-        for (int k = 0; k < 32; ++k) acc += lcol[k] * Y[(base + k) * YP + c];
-        Y[r * YP + c] -= acc;
-      }
-    }
-    __syncthreads();
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ void load_y(T* Y, const T* __restrict__ X, int n, int m, int c0, int w, int YP, int NP) {
-  for (int q = threadIdx.x; q < NP * w; q += kThreads) {
-    const int r = q / w, c = q - r * w;
-    Y[r * YP + c] = (r < n) ? X[(size_t)r * m + c0 + c] : T(0);
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ void store_y(T* __restrict__ X, const T* Y, int n, int m, int c0, int w, int YP) {
-  for (int q = threadIdx.x; q < n * w; q += kThreads) {
-    const int r = q / w, c = q - r * w;
-    X[(size_t)r * m + c0 + c] = Y[r * YP + c];
-  }
-}
-
+// y_j <- Linv_j y_j for the nodes j = s-1+2s*t of one level.
+// out[r] = y[r] / L[r][r] + sum_{k<r} U[k][r] y[k]; thread (r, part) reads
+// U[k][r] for its K part (coalesced in r); parts are summed through LDS.
 template <typename T, int NB, int NF>
-__global__ __launch_bounds__(kThreads) void k_fwd_blk(const T* __restrict__ L, const T* __restrict__ E,
-                                                     T* __restrict__ x, const uint8_t* __restrict__ mask, int N, int n_,
-                                                     int m, int nE, int s, int off, int first) {
+__global__ __launch_bounds__(kThreads) void k_apply_fwd(const T* __restrict__ L, T* __restrict__ x,
+                                                       const uint8_t* __restrict__ mask, int N, int n_, int m,
+                                                       int s) {
   const int n = NF ? NF : n_;
   constexpr int NP = 32 * NB;
-  constexpr int YP = kRhsChunk + 1;
+  constexpr int PARTS = kThreads / NP;
+  constexpr int KP = (NP + PARTS - 1) / PARTS;
   const int b = blockIdx.z;
   if (masked_out(mask, b)) return;
-  const int u = blockIdx.x;
+  const int j = s - 1 + 2 * s * blockIdx.x;
   const int c0 = blockIdx.y * kRhsChunk;
   const int w = min(kRhsChunk, m - c0);
-  __shared__ T Y0[NP * YP];
-  __shared__ T Y1[NP * YP];
-  __shared__ T Y2[NP * YP];
-  const size_t nn = (size_t)n * n;
-  const T* Eb = E + (size_t)b * nE * nn;
-  T* xb = x + (size_t)b * N * n * m;
-  int j;
-  bool do_solve;
-  if (first) {
-    j = 2 * u;
-    do_solve = true;
-  } else {
-    j = 2 * s * (u + 1) - 1;
-    do_solve = (u % 2) == 0;
+  const size_t nn = (size_t)n * n, nm = (size_t)n * m;
+  const T* U = L + ((size_t)b * N + j) * nn;
+  T* y = x + ((size_t)b * N + j) * nm;
+  __shared__ T Y[NP * kRP];
+  __shared__ T Pt[PARTS * NP * kRP];
+  const int tid = threadIdx.x;
+  const int r = tid % NP, part = tid / NP;
+  const bool act = part < PARTS;
+  T a[KP];
+#pragma unroll
+  for (int e = 0; e < KP; ++e) {
+    const int k = part * KP + e;
+    const T g = U[(size_t)min(k, n - 1) * n + min(r, n - 1)];
+    a[e] = (act && k < r && r < n) ? g : T(0);
   }
-  load_y(Y0, xb + (size_t)j * n * m, n, m, c0, w, YP, NP);
-  if (!first) {
-    const bool right = j + s < N;
-    load_y(Y1, xb + (size_t)(j - s) * n * m, n, m, c0, w, YP, NP);
-    if (right) load_y(Y2, xb + (size_t)(j + s) * n * m, n, m, c0, w, YP, NP);
-    __syncthreads();
-    // y_j -= R y_{j-s}, R = block (j, j-s)
-    panel_sub_AY<T, NB>(Y0, Eb + (size_t)(off + (j - s) / s) * nn, Y1, n, w, YP);
-    __syncthreads();
-    // y_j -= Lc^T y_{j+s}, Lc = block (j+s, j)
-    if (right) panel_sub_AtY<T, NB>(Y0, Eb + (size_t)(off + j / s) * nn, Y2, n, w, YP);
+  const T dg = U[(size_t)min(r, n - 1) * n + min(r, n - 1)];
+  for (int q = tid; q < NP * w; q += kThreads) {
+    const int k = q / w, c = q - k * w;
+    Y[k * kRP + c] = (k < n) ? y[(size_t)k * m + c0 + c] : T(0);
   }
   __syncthreads();
-  if (do_solve) panel_lower_solve<T, NB>(Y0, L + ((size_t)b * N + j) * nn, n, w, YP);
-  store_y(xb + (size_t)j * n * m, Y0, n, m, c0, w, YP);
+  if (act) {
+    for (int c = 0; c < w; ++c) {
+      T acc = T(0);
+#pragma unroll
+      for (int e = 0; e < KP; ++e) {
+        const int k = part * KP + e;
+        if (k < NP) acc += a[e] * Y[k * kRP + c];
+      }
+      Pt[(part * NP + r) * kRP + c] = acc;
+    }
+  }
+  __syncthreads();
+  if (part == 0 && r < n) {
+    for (int c = 0; c < w; ++c) {
+      T acc = Y[r * kRP + c] / dg;
+      for (int p = 0; p < PARTS; ++p) acc += Pt[(p * NP + r) * kRP + c];
+      y[(size_t)r * m + c0 + c] = acc;
+    }
+  }
 }
 
+// x_i <- Linv_i^T x_i for the nodes of one level.
+// out[r] = x[r] / L[r][r] + sum_{k>r} U[r][k] x[k]: 8 lanes per row read
+// contiguous slices of row r of U, then a width-8 shuffle reduction.
 template <typename T, int NB, int NF>
-__global__ __launch_bounds__(kThreads) void k_bwd_blk(const T* __restrict__ L, const T* __restrict__ E,
-                                                     T* __restrict__ x, const uint8_t* __restrict__ mask, int N, int n_,
-                                                     int m, int nE, int s, int off) {
+__global__ __launch_bounds__(kThreads) void k_apply_bwd(const T* __restrict__ L, T* __restrict__ x,
+                                                       const uint8_t* __restrict__ mask, int N, int n_, int m,
+                                                       int s) {
   const int n = NF ? NF : n_;
   constexpr int NP = 32 * NB;
-  constexpr int YP = kRhsChunk + 1;
+  constexpr int KPL = NP / 8;       // contiguous elements per lane
+  constexpr int PASSES = NP / 32;   // 32 rows per pass (8 lanes each)
   const int b = blockIdx.z;
   if (masked_out(mask, b)) return;
   const int i = s - 1 + 2 * s * blockIdx.x;
   const int c0 = blockIdx.y * kRhsChunk;
   const int w = min(kRhsChunk, m - c0);
-  __shared__ T Y0[NP * YP];
-  __shared__ T Y1[NP * YP];
-  __shared__ T Y2[NP * YP];
-  const size_t nn = (size_t)n * n;
-  const T* Eb = E + (size_t)b * nE * nn;
-  T* xb = x + (size_t)b * N * n * m;
-  const bool right = i + s < N, left = i >= s;
-  load_y(Y0, xb + (size_t)i * n * m, n, m, c0, w, YP, NP);
-  if (right) load_y(Y1, xb + (size_t)(i + s) * n * m, n, m, c0, w, YP, NP);
-  if (left) load_y(Y2, xb + (size_t)(i - s) * n * m, n, m, c0, w, YP, NP);
+  const size_t nn = (size_t)n * n, nm = (size_t)n * m;
+  const T* U = L + ((size_t)b * N + i) * nn;
+  T* y = x + ((size_t)b * N + i) * nm;
+  __shared__ T Y[NP * kRP];
+  const int tid = threadIdx.x;
+  const int sub = tid & 7, rr = tid >> 3;
+  T a[PASSES][KPL];
+#pragma unroll
+  for (int p = 0; p < PASSES; ++p) {
+    const int r = rr + 32 * p;
+#pragma unroll
+    for (int e = 0; e < KPL; ++e) {
+      const int k = sub * KPL + e;
+      const T g = U[(size_t)min(r, n - 1) * n + min(k, n - 1)];
+      a[p][e] = (r < n && k < n && k > r) ? g : T(0);
+    }
+  }
+  for (int q = tid; q < NP * w; q += kThreads) {
+    const int k = q / w, c = q - k * w;
+    Y[k * kRP + c] = (k < n) ? y[(size_t)k * m + c0 + c] : T(0);
+  }
   __syncthreads();
-  // x_i -= R^T x_{i+s}, R = block (i+s, i)
-  if (right) panel_sub_AtY<T, NB>(Y0, Eb + (size_t)(off + i / s) * nn, Y1, n, w, YP);
-  __syncthreads();
-  // x_i -= Lc x_{i-s}, Lc = block (i, i-s)
-  if (left) panel_sub_AY<T, NB>(Y0, Eb + (size_t)(off + (i - s) / s) * nn, Y2, n, w, YP);
-  __syncthreads();
-  panel_upper_solve_T<T, NB>(Y0, L + ((size_t)b * N + i) * nn, n, w, YP);
-  store_y(xb + (size_t)i * n * m, Y0, n, m, c0, w, YP);
+#pragma unroll
+  for (int p = 0; p < PASSES; ++p) {
+    const int r = rr + 32 * p;
+    const T dg = (sub == 0 && r < n) ? U[(size_t)r * n + r] : T(1);
+    for (int c = 0; c < w; ++c) {
+      T acc = T(0);
+#pragma unroll
+      for (int e = 0; e < KPL; ++e) acc += a[p][e] * Y[(sub * KPL + e) * kRP + c];
+#pragma unroll
+      for (int o = 4; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 8);
+      if (sub == 0 && r < n) y[(size_t)r * m + c0 + c] = acc + Y[r * kRP + c] / dg;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -660,11 +639,14 @@ __global__ __launch_bounds__(kThreads) void k_bwd_blk(const T* __restrict__ L, c
 // ---------------------------------------------------------------------------
 template <typename T, int NB>
 constexpr size_t potrf_blk_lds() {
-  return sizeof(T) * ((size_t)(32 * NB) * (32 * NB + 1) + 64);
+  return sizeof(T) * ((size_t)(32 * NB) * (32 * NB + 1) + 32 * NB + 32);
 }
 template <typename T, int NB>
 constexpr size_t trsm_blk_lds() {
-  return sizeof(T) * ((size_t)(NB * (NB + 1) / 2) * 32 * 33 + 32 * NB + (size_t)(32 * NB) * 33);
+  // upper tiles + max(R chunk 32 x (NP+1), Lc chunk NP x 33)
+  return sizeof(T) * ((size_t)(NB * (NB + 1) / 2) * 32 * 33 +
+                      ((size_t)32 * (32 * NB + 1) > (size_t)(32 * NB) * 33 ? (size_t)32 * (32 * NB + 1)
+                                                                             : (size_t)(32 * NB) * 33));
 }
 
 template <typename T, int NB>
@@ -683,6 +665,15 @@ static hipError_t set_blk_attrs() {
                              (int)trsm_blk_lds<T, NB>());
 }
 
+template <int NB>
+static hipError_t set_mixed_attrs() {
+  hipError_t e = hipFuncSetAttribute((const void*)k_potrf_blk<float, NB, 0, double>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)potrf_blk_lds<double, NB>());
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_potrf_blk<float, NB, 32 * NB, double>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)potrf_blk_lds<double, NB>());
+}
+
 hipError_t init_blocked_kernels() {
   hipError_t e;
   if ((e = set_blk_attrs<float, 1>()) != hipSuccess) return e;
@@ -693,19 +684,31 @@ hipError_t init_blocked_kernels() {
   if ((e = set_blk_attrs<double, 2>()) != hipSuccess) return e;
   if ((e = set_blk_attrs<double, 3>()) != hipSuccess) return e;
   if ((e = set_blk_attrs<double, 4>()) != hipSuccess) return e;
+  if ((e = set_mixed_attrs<1>()) != hipSuccess) return e;
+  if ((e = set_mixed_attrs<2>()) != hipSuccess) return e;
+  if ((e = set_mixed_attrs<3>()) != hipSuccess) return e;
+  if ((e = set_mixed_attrs<4>()) != hipSuccess) return e;
   return hipSuccess;
 }
 
 template <typename T, int NB, int NF>
-static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, hipStream_t st) {
+static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, bool f64_pivots,
+                                  hipStream_t st) {
   const int nE = (int)off_diag_storage_len(N);
   const int Lv = recursive_iterations(N);
   const Level lv = level_info(N, lidx);
   if (lv.dim <= 0 || B <= 0) return hipSuccess;
-  const size_t lds_potrf = potrf_blk_lds<T, NB>();
   const size_t lds_trsm = trsm_blk_lds<T, NB>();
   kt_begin(kKcPotrf, st);
-  hipLaunchKernelGGL((k_potrf_blk<T, NB, NF>), dim3(lv.dim, B), dim3(kThreads), lds_potrf, st, L, mask, N, n, lv.s);
+  if (f64_pivots && sizeof(T) == 4) {
+    const size_t lds_potrf = potrf_blk_lds<double, NB>();
+    hipLaunchKernelGGL((k_potrf_blk<T, NB, NF, double>), dim3(lv.dim, B), dim3(kThreads), lds_potrf, st, L, mask, N,
+                       n, lv.s);
+  } else {
+    const size_t lds_potrf = potrf_blk_lds<T, NB>();
+    hipLaunchKernelGGL((k_potrf_blk<T, NB, NF, T>), dim3(lv.dim, B), dim3(kThreads), lds_potrf, st, L, mask, N, n,
+                       lv.s);
+  }
   kt_end(kKcPotrf, st);
   if (lidx == Lv - 1) return hipGetLastError();
   kt_begin(kKcTrsm, st);
@@ -729,19 +732,21 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
 }
 
 template <typename T>
-hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, hipStream_t st) {
+hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, bool f64_pivots,
+                                hipStream_t st) {
+  const bool fp = f64_pivots;
   switch (n) {
-    case 32: return factor_level_nb<T, 1, 32>(L, E, mask, B, N, n, lidx, st);
-    case 64: return factor_level_nb<T, 2, 64>(L, E, mask, B, N, n, lidx, st);
-    case 96: return factor_level_nb<T, 3, 96>(L, E, mask, B, N, n, lidx, st);
-    case 128: return factor_level_nb<T, 4, 128>(L, E, mask, B, N, n, lidx, st);
+    case 32: return factor_level_nb<T, 1, 32>(L, E, mask, B, N, n, lidx, fp, st);
+    case 64: return factor_level_nb<T, 2, 64>(L, E, mask, B, N, n, lidx, fp, st);
+    case 96: return factor_level_nb<T, 3, 96>(L, E, mask, B, N, n, lidx, fp, st);
+    case 128: return factor_level_nb<T, 4, 128>(L, E, mask, B, N, n, lidx, fp, st);
     default: break;
   }
   switch ((n + 31) / 32) {
-    case 1: return factor_level_nb<T, 1, 0>(L, E, mask, B, N, n, lidx, st);
-    case 2: return factor_level_nb<T, 2, 0>(L, E, mask, B, N, n, lidx, st);
-    case 3: return factor_level_nb<T, 3, 0>(L, E, mask, B, N, n, lidx, st);
-    case 4: return factor_level_nb<T, 4, 0>(L, E, mask, B, N, n, lidx, st);
+    case 1: return factor_level_nb<T, 1, 0>(L, E, mask, B, N, n, lidx, fp, st);
+    case 2: return factor_level_nb<T, 2, 0>(L, E, mask, B, N, n, lidx, fp, st);
+    case 3: return factor_level_nb<T, 3, 0>(L, E, mask, B, N, n, lidx, fp, st);
+    case 4: return factor_level_nb<T, 4, 0>(L, E, mask, B, N, n, lidx, fp, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -751,19 +756,19 @@ static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* m
                                   int step, hipStream_t st) {
   const int nE = (int)off_diag_storage_len(N);
   const int mch = (m + kRhsChunk - 1) / kRhsChunk;
-  int items, s = 0, off = 0;
-  if (step < 0) {
-    items = (N + 1) / 2;
-  } else {
-    const Level lv = level_info(N, step);
-    s = lv.s;
-    off = lv.off;
-    items = N / (2 * s);
-  }
-  if (items <= 0 || B <= 0 || mch <= 0) return hipSuccess;
+  if (B <= 0 || mch <= 0) return hipSuccess;
   kt_begin(kKcForward, st);
-  hipLaunchKernelGGL((k_fwd_blk<T, NB, NF>), dim3(items, mch, B), dim3(kThreads), 0, st, L, E, x, mask, N, n, m, nE, s, off,
-                     step < 0 ? 1 : 0);
+  if (step >= 0) {
+    const Level lv = level_info(N, step);
+    const int items = N / (2 * lv.s);
+    if (items > 0)
+      hipLaunchKernelGGL((k_sweep_update<T, NB, NF, false>), dim3(items, NB * mch, B), dim3(kThreads), 0, st, E,
+                         x, mask, N, n, m, nE, lv.s, lv.off);
+  }
+  const Level nx = level_info(N, step + 1);
+  if (nx.dim > 0)
+    hipLaunchKernelGGL((k_apply_fwd<T, NB, NF>), dim3(nx.dim, mch, B), dim3(kThreads), 0, st, L, x, mask, N, n, m,
+                       nx.s);
   kt_end(kKcForward, st);
   return hipGetLastError();
 }
@@ -795,8 +800,12 @@ static hipError_t backward_level_nb(const T* L, const T* E, T* x, const uint8_t*
   const Level lv = level_info(N, lidx);
   if (lv.dim <= 0 || B <= 0 || mch <= 0) return hipSuccess;
   kt_begin(kKcBackward, st);
-  hipLaunchKernelGGL((k_bwd_blk<T, NB, NF>), dim3(lv.dim, mch, B), dim3(kThreads), 0, st, L, E, x, mask, N, n, m, nE,
-                     lv.s, lv.off);
+  // the top node has no neighbours (i = s-1, i+s >= N)
+  if (!(lv.dim == 1 && lv.s - 1 + lv.s >= N))
+    hipLaunchKernelGGL((k_sweep_update<T, NB, NF, true>), dim3(lv.dim, NB * mch, B), dim3(kThreads), 0, st, E, x,
+                       mask, N, n, m, nE, lv.s, lv.off);
+  hipLaunchKernelGGL((k_apply_bwd<T, NB, NF>), dim3(lv.dim, mch, B), dim3(kThreads), 0, st, L, x, mask, N, n, m,
+                     lv.s);
   kt_end(kKcBackward, st);
   return hipGetLastError();
 }
@@ -820,8 +829,10 @@ hipError_t backward_level_blocked(const T* L, const T* E, T* x, const uint8_t* m
   }
 }
 
-template hipError_t factor_level_blocked<float>(float*, float*, const uint8_t*, int, int, int, int, hipStream_t);
-template hipError_t factor_level_blocked<double>(double*, double*, const uint8_t*, int, int, int, int, hipStream_t);
+template hipError_t factor_level_blocked<float>(float*, float*, const uint8_t*, int, int, int, int, bool,
+                                                hipStream_t);
+template hipError_t factor_level_blocked<double>(double*, double*, const uint8_t*, int, int, int, int, bool,
+                                                 hipStream_t);
 template hipError_t forward_step_blocked<float>(const float*, const float*, float*, const uint8_t*, int, int, int, int,
                                                 int, hipStream_t);
 template hipError_t forward_step_blocked<double>(const double*, const double*, double*, const uint8_t*, int, int, int,

@@ -20,21 +20,12 @@ namespace socu {
 // Threads per workgroup for every level kernel: four 64-wide wavefronts.
 constexpr int kThreads = 256;
 
-// Largest block size n handled by the LDS-resident kernels.  A packed
-// lower triangle of a 128x128 fp64 block is 66 KiB of LDS.
+// Largest block size n handled by the LDS-resident kernels (a padded
+// 128x129 fp64 block is 129 KiB of the 160 KiB of LDS per workgroup).
 constexpr int kMaxN = 128;
 
 // Column chunk of the right-hand sides processed by one workgroup.
 constexpr int kRhsChunk = 16;
-
-// Output tile edge of the Schur-complement kernel.
-constexpr int kTile = 32;
-
-// Column/row chunk of a triangular-solve workgroup in the factor.
-constexpr int kTrsmChunk = 32;
-
-// K-chunk of the LDS-staged block products.
-constexpr int kKChunk = 32;
 
 struct Level {
   int l;         // level index
@@ -68,9 +59,6 @@ __host__ inline Level level_info(int N, int l) {
   v.next_off = off + (N >> l) - 1;
   return v;
 }
-
-// Index of the first element of a packed lower-triangular row r.
-__device__ __forceinline__ int tri(int r) { return (r * (r + 1)) >> 1; }
 
 __device__ __forceinline__ float rsqrt_t(float x) { return rsqrtf(x); }
 __device__ __forceinline__ double rsqrt_t(double x) { return rsqrt(x); }

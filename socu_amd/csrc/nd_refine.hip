@@ -43,15 +43,11 @@ __global__ __launch_bounds__(kThreads) void k_zero_f32(float* __restrict__ dst, 
     d[i] = 0.f;
 }
 
-// Refinement state per batch entry: [0] = ||b||^2, [1] = ||r||^2 of the last
-// residual, [2] = done flag (1.0 when converged or masked out).
-struct RefineState {
-  double* state;     // B x 3
-  double* partial;   // B x nblk partial sums of squares
-  int nblk;          // residual workgroups per batch entry
-};
-
-__device__ __forceinline__ bool refine_done(const double* state, int b) { return state[3 * b + 2] != 0.0; }
+// Refinement state per batch entry (4 doubles): [0] = ||b||^2, [1] = ||r||^2
+// of the last residual, [2] = done flag (1 when converged or masked out),
+// [3] = correction solves applied.  active[b] = !done, as a byte mask for the
+// FP32 sweep kernels.
+__device__ __forceinline__ bool refine_done(const double* state, int b) { return state[4 * b + 2] != 0.0; }
 
 // x (FP64) <- 0 and r32 <- b (FP32); records ||b||^2 partials.
 __global__ __launch_bounds__(kThreads) void k_refine_init(double* __restrict__ x, const double* __restrict__ b_in,
@@ -77,16 +73,19 @@ __global__ __launch_bounds__(kThreads) void k_refine_init(double* __restrict__ x
   if (threadIdx.x == 0) partial[(size_t)b * nblk + blockIdx.x] = red[0];
 }
 
-// state[b] <- (sum partials, 0, done = masked_out); run with one thread per b.
-__global__ void k_refine_start(double* __restrict__ state, const double* __restrict__ partial, int nblk, int B,
-                               const uint8_t* __restrict__ mask) {
+// state[b] <- (||b||^2, ||b||^2, done = masked out, 0); one thread per b.
+__global__ void k_refine_start(double* __restrict__ state, uint8_t* __restrict__ active,
+                               const double* __restrict__ partial, int nblk, int B, const uint8_t* __restrict__ mask) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   double s = 0.0;
   for (int q = 0; q < nblk; ++q) s += partial[(size_t)b * nblk + q];
-  state[3 * b + 0] = s;
-  state[3 * b + 1] = s;
-  state[3 * b + 2] = masked_out(mask, b) ? 1.0 : 0.0;
+  const bool off = masked_out(mask, b);
+  state[4 * b + 0] = s;
+  state[4 * b + 1] = s;
+  state[4 * b + 2] = off ? 1.0 : 0.0;
+  state[4 * b + 3] = 0.0;
+  active[b] = off ? 0 : 1;
 }
 
 // x += d (FP32 correction, converted), for systems not yet converged.
@@ -100,9 +99,14 @@ __global__ __launch_bounds__(kThreads) void k_refine_update(double* __restrict__
 
 // r = b - Psi x for block row i (FP64), one workgroup per (row, batch):
 //   r_i = b_i - sym(L_i) x_i - E_{i-1} x_{i-1} - E_i^T x_{i+1}
-// Only tril(L_i) is read (the matrix the reference's solver sees).  The FP32
-// copy r32 feeds the next correction solve; partial[b][i] gets ||r_i||^2.
-template <int M_MAX>
+// Only tril(L_i) is read (the matrix the reference's solver sees).
+// Row-access products (tril(L_i) x_i, E_{i-1} x_{i-1}) use 8 lanes per row
+// and a shuffle reduction; column-access products (strict upper of sym(L_i)
+// from the lower triangle, E_i^T x_{i+1}) use one thread per row and K parts.
+// The FP32 copy of r feeds the next correction solve; partial[b][i] gets
+// ||r_i||^2.
+constexpr int kResM = 16;  // right-hand sides per residual workgroup pass
+
 __global__ __launch_bounds__(kThreads) void k_residual(const double* __restrict__ L, const double* __restrict__ E,
                                                       const double* __restrict__ x, const double* __restrict__ b_in,
                                                       float* __restrict__ r32, double* __restrict__ partial,
@@ -111,57 +115,191 @@ __global__ __launch_bounds__(kThreads) void k_residual(const double* __restrict_
   const int i = blockIdx.x;
   if (refine_done(state, b)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  double* xs = reinterpret_cast<double*>(smem_raw);  // 3 x n x m: x_{i-1}, x_i, x_{i+1}
-  __shared__ double red[kThreads];
   const size_t nn = (size_t)n * n, nm = (size_t)n * m;
+  double* xs = reinterpret_cast<double*>(smem_raw);  // 3 x n x m: x_{i-1}, x_i, x_{i+1}
+  double* acc_s = xs + 3 * nm;                        // n x m accumulated Psi x
+  __shared__ double red[kThreads];
   const double* xb = x + (size_t)b * N * nm;
   for (int q = threadIdx.x; q < 3 * (int)nm; q += kThreads) {
     const int which = q / (int)nm, e = q - which * (int)nm;
     const int row = i - 1 + which;
     xs[q] = (row >= 0 && row < N) ? xb[(size_t)row * nm + e] : 0.0;
   }
+  for (int q = threadIdx.x; q < (int)nm; q += kThreads) acc_s[q] = 0.0;
   __syncthreads();
   const double* Li = L + ((size_t)b * N + i) * nn;
   const double* Eb = E + (size_t)b * nE * nn;
-  double acc = 0.0;
-  // thread -> (r, c) output; rows split across threads, each thread walks k.
-  for (int q = threadIdx.x; q < (int)nm; q += kThreads) {
-    const int r = q / m, c = q - r * m;
-    double s = b_in[(size_t)b * N * nm + (size_t)i * nm + q];
-    for (int k = 0; k < n; ++k) {
-      const double lik = (k <= r) ? Li[(size_t)r * n + k] : Li[(size_t)k * n + r];
-      s -= lik * xs[nm + (size_t)k * m + c];
+  const double* Em = (i > 0) ? Eb + (size_t)(i - 1) * nn : nullptr;     // block (i, i-1)
+  const double* Ep = (i + 1 < N) ? Eb + (size_t)i * nn : nullptr;       // block (i+1, i)
+  const double* x_prev = xs;
+  const double* x_cur = xs + nm;
+  const double* x_next = xs + 2 * nm;
+  const int tid = threadIdx.x;
+  // (1) row products: 8 lanes per row
+  {
+    const int sub = tid & 7;
+    for (int r = tid >> 3; r < n; r += kThreads / 8) {
+      for (int c0 = 0; c0 < m; c0 += kResM) {
+        const int w = min(kResM, m - c0);
+        double acc[kResM];
+#pragma unroll
+        for (int c = 0; c < kResM; ++c) acc[c] = 0.0;
+        for (int k = sub; k < n; k += 8) {
+          const double lv = (k <= r) ? Li[(size_t)r * n + k] : 0.0;
+          const double ev = Em ? Em[(size_t)r * n + k] : 0.0;
+#pragma unroll
+          for (int c = 0; c < kResM; ++c)
+            if (c < w) acc[c] += lv * x_cur[(size_t)k * m + c0 + c] + ev * x_prev[(size_t)k * m + c0 + c];
+        }
+#pragma unroll
+        for (int c = 0; c < kResM; ++c) {
+          double v = acc[c];
+          for (int o = 4; o >= 1; o >>= 1) v += __shfl_xor(v, o, 8);
+          if (sub == 0 && c < w) acc_s[(size_t)r * m + c0 + c] += v;
+        }
+      }
     }
-    if (i > 0) {
-      const double* Em = Eb + (size_t)(i - 1) * nn;  // block (i, i-1)
-      for (int k = 0; k < n; ++k) s -= Em[(size_t)r * n + k] * xs[(size_t)k * m + c];
-    }
-    if (i + 1 < N) {
-      const double* Ep = Eb + (size_t)i * nn;  // block (i+1, i): contributes E_i^T x_{i+1}
-      for (int k = 0; k < n; ++k) s -= Ep[(size_t)k * n + r] * xs[2 * nm + (size_t)k * m + c];
-    }
-    r32[(size_t)b * N * nm + (size_t)i * nm + q] = (float)s;
-    acc += s * s;
   }
-  red[threadIdx.x] = acc;
+  __syncthreads();
+  // (2) column products: thread per row r, loop over k (reads coalesced in r)
+  for (int r = tid; r < n; r += kThreads) {
+    for (int c0 = 0; c0 < m; c0 += kResM) {
+      const int w = min(kResM, m - c0);
+      double acc[kResM];
+#pragma unroll
+      for (int c = 0; c < kResM; ++c) acc[c] = 0.0;
+      for (int k = 0; k < n; ++k) {
+        const double lv = (k > r) ? Li[(size_t)k * n + r] : 0.0;
+        const double ev = Ep ? Ep[(size_t)k * n + r] : 0.0;
+#pragma unroll
+        for (int c = 0; c < kResM; ++c)
+          if (c < w) acc[c] += lv * x_cur[(size_t)k * m + c0 + c] + ev * x_next[(size_t)k * m + c0 + c];
+      }
+#pragma unroll
+      for (int c = 0; c < kResM; ++c)
+        if (c < w) acc_s[(size_t)r * m + c0 + c] += acc[c];
+    }
+  }
+  __syncthreads();
+  double sq = 0.0;
+  const double* bi = b_in + (size_t)b * N * nm + (size_t)i * nm;
+  float* ri = r32 + (size_t)b * N * nm + (size_t)i * nm;
+  for (int q = tid; q < (int)nm; q += kThreads) {
+    const double v = bi[q] - acc_s[q];
+    ri[q] = (float)v;
+    sq += v * v;
+  }
+  red[tid] = sq;
   __syncthreads();
   for (int o = kThreads / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    if (tid < o) red[tid] += red[tid + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[(size_t)b * N + i] = red[0];
+  if (tid == 0) partial[(size_t)b * N + i] = red[0];
 }
 
 // ||r||^2 from the partials; sets the done flag when ||r|| <= tol ||b||.
-__global__ void k_refine_check(double* __restrict__ state, const double* __restrict__ partial, int nblk, int B,
-                               double tol2) {
+__global__ void k_refine_check(double* __restrict__ state, uint8_t* __restrict__ active,
+                               const double* __restrict__ partial, int nblk, int B, double tol2) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
-  if (state[3 * b + 2] != 0.0) return;
+  if (state[4 * b + 2] != 0.0) return;
   double s = 0.0;
   for (int q = 0; q < nblk; ++q) s += partial[(size_t)b * nblk + q];
-  state[3 * b + 1] = s;
-  if (s <= tol2 * state[3 * b + 0]) state[3 * b + 2] = 1.0;
+  state[4 * b + 1] = s;
+  state[4 * b + 3] += 1.0;
+  if (s <= tol2 * state[4 * b + 0] || !(s == s)) {  // converged, or NaN (non-SPD input)
+    state[4 * b + 2] = 1.0;
+    active[b] = 0;
+  }
+}
+
+}  // namespace socu
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+namespace socu {
+
+static int refine_blocks(size_t count) {
+  const size_t blk = (count + kThreads - 1) / kThreads;
+  return (int)(blk < 256 ? (blk > 0 ? blk : 1) : 256);
+}
+
+size_t mixed_residual_lds(int n, int m) { return sizeof(double) * 4 * (size_t)n * m; }
+
+hipError_t init_refine_kernels() {
+  return hipFuncSetAttribute((const void*)k_residual, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+hipError_t mixed_alloc(MixedPlan& p) {
+  const size_t nn = (size_t)p.n * p.n, nm = (size_t)p.n * p.m;
+  p.nE = (int)off_diag_storage_len(p.N);
+  p.nblk_init = refine_blocks((size_t)p.N * nm);
+  hipError_t e;
+  if ((e = hipMalloc(&p.L32, sizeof(float) * p.B * (size_t)p.N * nn)) != hipSuccess) return e;
+  if ((e = hipMalloc(&p.E32, sizeof(float) * p.B * (size_t)(p.nE > 0 ? p.nE : 1) * nn)) != hipSuccess) return e;
+  if ((e = hipMalloc(&p.d32, sizeof(float) * p.B * (size_t)p.N * nm)) != hipSuccess) return e;
+  if ((e = hipMalloc(&p.b64, sizeof(double) * p.B * (size_t)p.N * nm)) != hipSuccess) return e;
+  if ((e = hipMalloc(&p.state, sizeof(double) * 4 * p.B)) != hipSuccess) return e;
+  const int np = p.N > p.nblk_init ? p.N : p.nblk_init;
+  if ((e = hipMalloc(&p.partial, sizeof(double) * p.B * (size_t)np)) != hipSuccess) return e;
+  if ((e = hipMalloc(&p.active, (size_t)p.B)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
+void mixed_free(MixedPlan& p) {
+  void* ptrs[] = {p.L32, p.E32, p.d32, p.b64, p.state, p.partial, p.active};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  p.L32 = p.E32 = p.d32 = nullptr;
+  p.b64 = p.state = p.partial = nullptr;
+  p.active = nullptr;
+}
+
+hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
+  const int B = p.B, N = p.N, n = p.n, m = p.m, nE = p.nE;
+  const size_t nn = (size_t)n * n, nm = (size_t)n * m;
+  const size_t xcount = (size_t)N * nm;
+  hipError_t e = hipMemcpyAsync(p.b64, p.x, sizeof(double) * B * xcount, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return e;
+  // FP32 working copy of Psi; the fill-in region starts at zero.
+  const size_t lc = (size_t)N * nn;
+  hipLaunchKernelGGL(k_cast_f64_f32, dim3(refine_blocks(lc), B), dim3(kThreads), 0, st, p.L32, p.L, lc, lc, lc,
+                     p.mask);
+  if (N > 1) {
+    const size_t ec = (size_t)(N - 1) * nn, es = (size_t)nE * nn;
+    hipLaunchKernelGGL(k_cast_f64_f32, dim3(refine_blocks(ec), B), dim3(kThreads), 0, st, p.E32, p.E, ec, es, es,
+                       p.mask);
+    const size_t fc = (size_t)(nE - (N - 1)) * nn;
+    if (fc > 0)
+      hipLaunchKernelGGL(k_zero_f32, dim3(refine_blocks(fc), B), dim3(kThreads), 0, st, p.E32 + ec, fc, es, p.mask);
+  }
+  const int Lv = recursive_iterations(N);
+  for (int l = 0; l < Lv; ++l)
+    if ((e = factor_level_blocked<float>(p.L32, p.E32, p.mask, B, N, n, l, p.f64_pivots, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_refine_init, dim3(p.nblk_init, B), dim3(kThreads), 0, st, p.x, p.b64, p.d32, p.partial, xcount,
+                     p.nblk_init, p.mask);
+  hipLaunchKernelGGL(k_refine_start, dim3((B + 63) / 64), dim3(64), 0, st, p.state, p.active, p.partial, p.nblk_init,
+                     B, p.mask);
+  const size_t rlds = mixed_residual_lds(n, m);
+  for (int it = 0; it < p.max_iter; ++it) {
+    for (int step = -1; step < Lv - 1; ++step)
+      if ((e = forward_step_blocked<float>(p.L32, p.E32, p.d32, p.active, B, N, n, m, step, st)) != hipSuccess)
+        return e;
+    for (int l = Lv - 1; l >= 0; --l)
+      if ((e = backward_level_blocked<float>(p.L32, p.E32, p.d32, p.active, B, N, n, m, l, st)) != hipSuccess)
+        return e;
+    kt_begin(kKcRefine, st);
+    hipLaunchKernelGGL(k_refine_update, dim3(refine_blocks(xcount), B), dim3(kThreads), 0, st, p.x, p.d32, xcount,
+                       p.state);
+    hipLaunchKernelGGL(k_residual, dim3(N, B), dim3(kThreads), rlds, st, p.L, p.E, p.x, p.b64, p.d32, p.partial,
+                       p.state, N, n, m, nE);
+    hipLaunchKernelGGL(k_refine_check, dim3((B + 63) / 64), dim3(64), 0, st, p.state, p.active, p.partial, N, B,
+                       p.tol * p.tol);
+    kt_end(kKcRefine, st);
+  }
+  return hipGetLastError();
 }
 
 }  // namespace socu

@@ -84,6 +84,23 @@ __device__ __forceinline__ void wave_gemm(typename Mfma<T>::acc_t acc[2][2], con
   }
 }
 
+// C(32x32) += A(32xK) B(Kx32) with element fetch functors fa(i, k), fb(k, j)
+// (for operands with implicit structure, e.g. triangular tiles).
+template <typename T, typename FA, typename FB>
+__device__ __forceinline__ void wave_gemm_f(typename Mfma<T>::acc_t acc[2][2], FA fa, FB fb, int K) {
+  const int l = lane_id();
+  const int i = l & 15, kq = l >> 4;
+  for (int k = 0; k < K; k += 4) {
+    const int kk = k + kq;
+    const T a0 = fa(i, kk), a1 = fa(i + 16, kk);
+    const T b0 = fb(kk, i), b1 = fb(kk, i + 16);
+    acc[0][0] = Mfma<T>::mma(a0, b0, acc[0][0]);
+    acc[0][1] = Mfma<T>::mma(a0, b1, acc[0][1]);
+    acc[1][0] = Mfma<T>::mma(a1, b0, acc[1][0]);
+    acc[1][1] = Mfma<T>::mma(a1, b1, acc[1][1]);
+  }
+}
+
 // One 16x16 tile: C += A(16xK) B(Kx16), same stride convention.
 template <typename T>
 __device__ __forceinline__ void wave_gemm16(typename Mfma<T>::acc_t& acc, const T* A0, int ai, int ak, const T* B0,

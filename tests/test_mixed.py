@@ -1,0 +1,96 @@
+"""Mixed-precision factor_and_solve with FP64 refinement (an extension; the
+reference has no such path).  Checks: the FP64 relative residual reaches the
+requested tolerance, the solution matches the oracle's FP64 solution to the
+accuracy the residual implies, Psi (L, E) is untouched, masked entries are
+untouched, and graph replays agree with eager calls."""
+import numpy as np
+import pytest
+
+from oracle import nd
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(n, N, m, alpha, seed, B=1):
+    Ls, Es, bs = [], [], []
+    for k in range(B):
+        L0, E0, b0, _, _ = nd.readme_problem(n, N, m, alpha=alpha, seed=seed + k)
+        Ls.append(L0)
+        Es.append(E0)
+        bs.append(b0)
+    return np.stack(Ls), np.stack(Es), np.stack(bs)
+
+
+@pytest.mark.parametrize("n,N,m,alpha", [(32, 20, 1, 10.0), (64, 33, 4, 10.0), (128, 16, 1, 16.0), (36, 7, 3, 10.0),
+                                         (128, 64, 8, 16.0), (10, 100, 2, 10.0)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_mixed_reaches_fp64_residual(n, N, m, alpha, graph, device):
+    import torch
+    from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
+
+    L0, E0, b0 = _problem(n, N, m, alpha, seed=11, B=2)
+    L = torch.tensor(L0, device=device)
+    E = torch.tensor(E0, device=device)
+    x = torch.tensor(b0, device=device)
+    run = create_cholesky_mixed_factor_and_solve_launch(L, E, x, tol=1e-10, max_iter=8, use_cuda_graph=graph)
+    for rep in range(2):  # second call: graph replay / re-run on fresh b
+        x.copy_(torch.tensor(b0, device=device))
+        run()
+        rr, its = run.stats()
+        xg = x.cpu().numpy()
+        res = nd.rel_residual(L0, E0, xg, b0)
+        assert np.all(res <= 1e-10), (res, its)
+        np.testing.assert_allclose(res, rr, rtol=1e-6, atol=1e-16)
+        assert np.all(its >= 1) and np.all(its <= 8)
+        Lr, Er, xr = L0.copy(), E0.copy(), b0.copy()
+        nd.factor_and_solve(Lr, Er, xr)
+        assert np.linalg.norm(xg - xr) / np.linalg.norm(xr) < 1e-8
+    # Psi untouched
+    np.testing.assert_array_equal(L.cpu().numpy(), L0)
+    np.testing.assert_array_equal(E.cpu().numpy(), E0)
+
+
+def test_mixed_mask(device):
+    import torch
+    from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
+
+    L0, E0, b0 = _problem(32, 9, 2, 10.0, seed=3, B=4)
+    L = torch.tensor(L0, device=device)
+    E = torch.tensor(E0, device=device)
+    x = torch.tensor(b0, device=device)
+    mask = torch.tensor([True, False, True, False], device=device)
+    run = create_cholesky_mixed_factor_and_solve_launch(L, E, x, mask, use_cuda_graph=True)
+    run()
+    xg = x.cpu().numpy()
+    res = nd.rel_residual(L0, E0, xg, b0)
+    assert res[0] <= 1e-10 and res[2] <= 1e-10
+    np.testing.assert_array_equal(xg[1], b0[1])
+    np.testing.assert_array_equal(xg[3], b0[3])
+
+
+def test_mixed_needs_fewer_than_max_iter_for_well_conditioned(device):
+    import torch
+    from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
+
+    L0, E0, b0 = _problem(128, 32, 1, 16.0, seed=0)
+    x = torch.tensor(b0, device=device)
+    run = create_cholesky_mixed_factor_and_solve_launch(torch.tensor(L0, device=device),
+                                                        torch.tensor(E0, device=device), x, max_iter=10)
+    run()
+    rr, its = run.stats()
+    assert rr[0] <= 1e-10 and its[0] <= 3, (rr, its)
+
+
+def test_mixed_non_spd_stops_without_hanging(device):
+    import torch
+    from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
+
+    L0, E0, b0 = _problem(8, 5, 1, 10.0, seed=1)
+    L0[0, 2] = -np.eye(8)
+    x = torch.tensor(b0, device=device)
+    run = create_cholesky_mixed_factor_and_solve_launch(torch.tensor(L0, device=device),
+                                                        torch.tensor(E0, device=device), x, max_iter=4)
+    run()
+    rr, its = run.stats()
+    assert not (rr[0] <= 1e-10)
+    assert its[0] <= 4
