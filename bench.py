@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--max-iter", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the batched and gradient workloads")
+    ap.add_argument("--mpc-count", type=int, default=4096)
+    ap.add_argument("--secondary-steps", type=int, default=20)
     return ap.parse_args()
 
 
@@ -102,6 +105,114 @@ def _time_steps(step, restore, st, steps, warmup):
     return per, wall
 
 
+def bench_mpc_batch(args, dev, st):
+    """The uneven MPC batch: 4096 independent systems, N ~ U[10, 200],
+    n in {8, 12, 16, 24, 32}, m = 1, FP64 factor_and_solve, one tuned,
+    graph-captured batched launch per step.  HBM-bound: the roofline figure
+    is the compulsory bytes of every system's factor_and_solve over the step
+    time."""
+    import torch
+
+    from socu_amd import problems
+    from socu_amd.batched import create_batched_cholesky_launch
+
+    log(f"bench: generating the {args.mpc_count}-system uneven MPC batch")
+    batch = problems.mpc_batch(args.mpc_count, seed=1)
+    total = sum(L.size + E.size + b.size for L, E, b in batch)
+    host = np.empty(total)
+    spans = []
+    o = 0
+    for L, E, b in batch:
+        sp = []
+        for a in (L, E, b):
+            host[o:o + a.size] = a.ravel()
+            sp.append((o, a.shape))
+            o += a.size
+        spans.append(sp)
+    shapes = [(L.shape[0], L.shape[1]) for L, _, _ in batch]
+    del batch
+    pristine = torch.from_numpy(host).to(dev)
+    work = pristine.clone()
+
+    def views(t):
+        return [[t[o:o + int(np.prod(sh))].view(*sh) for o, sh in sp] for sp in spans]
+
+    wv = views(work)
+    Ls, Es, xs = [v[0] for v in wv], [v[1] for v in wv], [v[2] for v in wv]
+    t0 = time.perf_counter()
+    run = create_batched_cholesky_launch(Ls, Es, xs, tune=True, use_cuda_graph=True)
+    t_create = time.perf_counter() - t0
+    per, _ = _time_steps(run, lambda: work.copy_(pristine), st, args.secondary_steps, 3)
+    out_host = work.cpu().numpy()
+    worst = 0.0
+    for sp in spans:
+        (oL, shL), (oE, shE), (ob, shb) = sp
+        Lh = host[oL:oL + int(np.prod(shL))].reshape(shL)
+        Eh = host[oE:oE + int(np.prod(shE))].reshape(shE)
+        bh = host[ob:ob + int(np.prod(shb))].reshape(shb)
+        xh = out_host[ob:ob + int(np.prod(shb))].reshape(shb)
+        worst = max(worst, problems.rel_residual(Lh, Eh, xh, bh))
+    byt = sum(problems.nd_bytes(N, n, 1)[2] for N, n in shapes)
+    fl = sum(sum(problems.nd_flops(N, n, 1)) for N, n in shapes)
+    ms = float(per.mean())
+    gbs = byt / (ms * 1e-3) / 1e9
+    res = {
+        "workload": f"{args.mpc_count} uneven systems, N~U[10,200], n in {{8,12,16,24,32}}, m=1, FP64 "
+                    "factor_and_solve, one graph launch",
+        "ms_per_batch": round(ms, 4), "p50_ms": round(float(np.median(per)), 4),
+        "p99_ms": round(float(np.percentile(per, 99)), 4),
+        "solves_per_s": round(args.mpc_count / (ms * 1e-3), 1),
+        "gflops": round(fl / (ms * 1e-3) / 1e9, 2),
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 5), "algorithmic_bytes": byt},
+        "max_rel_resid": worst, "config": run.config(), "create_s_incl_tune": round(t_create, 3),
+    }
+    log(f"bench: mpc batch {res}")
+    return res
+
+
+def bench_adjoint(args, dev, st):
+    """Differentiable solve at B=256, n=32, N=128, m=4 (FP64): one step is
+    the forward factor_and_solve plus the backward transpose solve and
+    adjoint scatter."""
+    import torch
+
+    from socu_amd import problems
+    from socu_amd.autograd import block_tridiag_solve
+
+    B, N, n, m = 256, 128, 32, 4
+    g = torch.Generator(device=dev).manual_seed(2)
+    D = torch.tril(torch.randn(B, N, n, n, generator=g, device=dev, dtype=torch.float64))
+    D += 10 * torch.eye(n, device=dev, dtype=torch.float64)
+    Ec = torch.randn(B, N - 1, n, n, generator=g, device=dev, dtype=torch.float64)
+    L = torch.matmul(D, D.transpose(-1, -2).contiguous())
+    L[:, 1:] += torch.matmul(Ec, Ec.transpose(-1, -2).contiguous())
+    E = torch.zeros(B, problems.calculate_off_diag_storage_len(N), n, n, device=dev, dtype=torch.float64)
+    E[:, : N - 1] = torch.matmul(Ec, D[:, : N - 1].transpose(-1, -2).contiguous())
+    b = torch.randn(B, N, n, m, generator=g, device=dev, dtype=torch.float64)
+    w = torch.randn(B, N, n, m, generator=g, device=dev, dtype=torch.float64)
+    L.requires_grad_(True)
+    E.requires_grad_(True)
+    b.requires_grad_(True)
+
+    def step():
+        x = block_tridiag_solve(L, E, b)
+        torch.autograd.grad(x, (L, E, b), grad_outputs=w)
+
+    def fwd():
+        with torch.no_grad():
+            block_tridiag_solve(L, E, b)
+
+    per, _ = _time_steps(step, lambda: None, st, args.secondary_steps, 3)
+    perf, _ = _time_steps(fwd, lambda: None, st, args.secondary_steps, 3)
+    ms, msf = float(per.mean()), float(perf.mean())
+    res = {"workload": f"B={B} n={n} N={N} m={m} FP64, forward + backward (dL, dE, db)",
+           "ms_per_step": round(ms, 4), "forward_ms": round(msf, 4),
+           "backward_ms": round(ms - msf, 4), "systems_per_s": round(B / (ms * 1e-3), 1)}
+    log(f"bench: adjoint {res}")
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -111,8 +222,7 @@ def main():
 
     import torch
 
-    from oracle import nd
-    from socu_amd import _lib
+    from socu_amd import _lib, problems
     from socu_amd.block_tridiag_solver import create_cholesky_factor_and_solve_launch
     from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
 
@@ -122,7 +232,7 @@ def main():
     ncls = len(_lib.KERNEL_CLASSES)
     st = torch.cuda.current_stream(dev)
     log(f"bench: generating n={n} N={N} m={m} alpha={args.alpha}")
-    L0, E0, b0, _, _ = nd.readme_problem(n, N, m, alpha=args.alpha, seed=0)
+    L0, E0, b0 = problems.readme_problem(n, N, m, alpha=args.alpha, seed=0)
     Lp = torch.tensor(L0, device=dev)
     Ep = torch.tensor(E0, device=dev)
     bp = torch.tensor(b0, device=dev)
@@ -138,7 +248,7 @@ def main():
     fs = create_cholesky_factor_and_solve_launch(L, E, x, use_cuda_graph=True)
     per64, wall64 = _time_steps(fs, restore64, st, args.steps, args.warmup)
     ms64 = float(per64.mean())
-    res64 = float(nd.rel_residual(L0, E0, x.cpu().numpy(), b0))
+    res64 = problems.rel_residual(L0, E0, x.cpu().numpy(), b0)
     log(f"bench: fp64 factor_and_solve {ms64:.3f} ms (p50 {np.median(per64):.3f} p99 "
         f"{np.percentile(per64, 99):.3f}), rel-resid {res64:.2e}")
     k64, c64 = np.zeros(ncls), np.zeros(ncls, dtype=np.int64)
@@ -157,7 +267,7 @@ def main():
     perm, wallm = _time_steps(mx, lambda: xm.copy_(bp), st, args.steps, args.warmup)
     msm = float(perm.mean())
     rr, its = mx.stats()
-    resm = float(nd.rel_residual(L0, E0, xm.cpu().numpy(), b0))
+    resm = problems.rel_residual(L0, E0, xm.cpu().numpy(), b0)
     log(f"bench: mixed+refine {msm:.3f} ms (p50 {np.median(perm):.3f} p99 {np.percentile(perm, 99):.3f}), "
         f"rel-resid {resm:.2e} (device {rr[0]:.2e}) after {its[0]} correction solves")
     km_mixed, kc_mixed = _profile(lib, lambda a, b: _lib.check(lib.socu_mixed_profile(
@@ -171,7 +281,7 @@ def main():
     # --- the literal alpha=10 configuration (numerically singular at n=128) --
     literal = None
     if args.alpha != 10.0 and n == 128:
-        L1, E1, b1, _, _ = nd.readme_problem(n, N, m, alpha=10.0, seed=0)
+        L1, E1, b1 = problems.readme_problem(n, N, m, alpha=10.0, seed=0)
         L1t, E1t, x1 = torch.tensor(L1, device=dev), torch.tensor(E1, device=dev), torch.tensor(b1, device=dev)
         m1 = create_cholesky_mixed_factor_and_solve_launch(L1t, E1t, x1, tol=args.tol, max_iter=args.max_iter)
         m1()
@@ -180,13 +290,13 @@ def main():
         Lf, Ef = L1t.clone(), E1t.clone()
         create_cholesky_factor_and_solve_launch(Lf, Ef, xf)()
         literal = {"alpha": 10.0, "mixed_rel_resid": float(rr1[0]), "mixed_solves": int(it1[0]),
-                   "fp64_rel_resid": float(nd.rel_residual(L1, E1, xf.cpu().numpy(), b1))}
+                   "fp64_rel_resid": problems.rel_residual(L1, E1, xf.cpu().numpy(), b1)}
         log(f"bench: literal alpha=10: {literal}")
 
-    ff, fsv = nd.nd_flops(N, n, m)
+    ff, fsv = problems.nd_flops(N, n, m)
     # dominant kernel class of the measured (mixed) path; fp32 factor kernels
     dom = max(kernel_ms, key=kernel_ms.get)
-    cls_flops = _class_flops(N, n, m)
+    cls_flops = problems.class_flops(N, n, m)
     cls_flops["forward"] *= max(int(its[0]), 1)
     cls_flops["backward"] *= max(int(its[0]), 1)
     fp32_cls = dom in ("potrf", "trsm", "schur", "forward", "backward")
@@ -211,6 +321,10 @@ def main():
         cpu = {"value": round(rate, 4), "unit": "factor+solve/s", "cores": threads, "kind": "port",
                "sample": f"oracle/nd_oracle.c FP64 factor_and_solve (reaches rel-resid < 1e-13), same "
                          f"n={n} N={N} m={m} problem, {creps} runs"}
+
+    secondary = None
+    if not args.no_secondary:
+        secondary = {"mpc_batched": bench_mpc_batch(args, dev, st), "adjoint": bench_adjoint(args, dev, st)}
 
     mixed_ok = resm <= args.tol
     value = 1e3 / msm if mixed_ok else 0.0
@@ -240,26 +354,9 @@ def main():
                    "wall_ms_per_step_incl_restore": round(wallm * 1e3 / args.steps, 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "secondary": secondary,
     }
     print(json.dumps(out), flush=True)
-
-
-def _class_flops(N, n, m):
-    """Algorithmic flops per kernel class for one factor_and_solve."""
-    from oracle import nd
-
-    f = {"potrf": 0.0, "trsm": 0.0, "schur": 0.0, "forward": 0.0, "backward": 0.0, "refine": 0.0}
-    for lev in nd.level_schedule(N):
-        s = lev["s"]
-        for i in lev["nodes"]:
-            r = i + s < N
-            c = i >= s
-            f["potrf"] += n ** 3 / 3
-            f["trsm"] += (r + c) * n ** 3
-            f["schur"] += (r + c) * n ** 3 + (r and c) * 2 * n ** 3
-            f["forward"] += n * n * m + 2 * (r + c) * n * n * m
-            f["backward"] += n * n * m + 2 * (r + c) * n * n * m
-    return f
 
 
 if __name__ == "__main__":

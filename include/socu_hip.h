@@ -123,6 +123,53 @@ int socu_mixed_profile(socu_mixed* handle, void* stream, double* kernel_ms, int*
 int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
                     int dtype, void* stream, double* kernel_ms, int* kernel_launches);
 
+/* Batched uneven systems (no counterpart in the reference, whose launches
+ * batch only systems sharing N and n: block_tridiag_solver.py:749-768).
+ * Each entry is one independent system in the reference's layout: L (N,n,n),
+ * E (calculate_off_diag_storage_len(N),n,n) with a zero fill-in region, and x
+ * (N,n,m) (ignored for SOCU_OP_FACTOR); `reserved` must be 0.  Needs
+ * n <= socu_batched_max_block_size() and m <= 32.  mask is NULL or `count`
+ * bytes indexed like `systems`.  One workgroup per system walks all its
+ * levels; systems are grouped in tile classes (8, 16, 32 by max(n, m)), one
+ * launch per class, the classes on forked streams.  With `tune` the create
+ * call times each workgroup size (1, 2, 4 wavefronts) per class on a device
+ * copy of the batch (the caller's arrays are only read) and keeps the
+ * fastest; results are cached per process and, if the environment variable
+ * SOCU_TUNE_CACHE names a file, appended there and reused by later
+ * processes.  create copies the system table to the device and synchronises;
+ * execute allocates nothing and synchronises nothing.
+ * socu_batched_config returns per class the system count and wave count. */
+typedef struct socu_batch_system {
+  void* L;
+  void* E;
+  void* x;
+  int N;
+  int n;
+  int m;
+  int reserved;
+} socu_batch_system;
+typedef struct socu_batched socu_batched;
+int socu_batched_max_block_size(void);
+int socu_batched_create(socu_batched** out, int op, const socu_batch_system* systems, int count,
+                        const unsigned char* mask, int dtype, int tune, int use_graph);
+int socu_batched_execute(socu_batched* handle, void* stream);
+int socu_batched_config(socu_batched* handle, int* counts, int* waves);
+int socu_batched_destroy(socu_batched* handle);
+
+/* Adjoint scatter (no counterpart in the reference, which has no gradient).
+ * For x = Psi^{-1} b, with Psi the block-tridiagonal matrix of the solver
+ * (diagonal blocks from the lower triangle of L_i, couplings E_i at (i+1, i))
+ * and bbar = Psi^{-1} xbar (one solve with the same factor), writes the
+ * gradient of the loss with respect to the parameters:
+ *   dL_i = lower part of -(bbar_i x_i^T + x_i bbar_i^T), halved on the
+ *          diagonal, zero above it;
+ *   dE_e = -(bbar_{e+1} x_e^T + x_{e+1} bbar_e^T) for e < N-1, zero on the
+ *          fill-in region.
+ * Shapes as the solver: bbar, x (B,N,n,m); dL (B,N,n,n); dE (B,n_E,n,n).
+ * Masked-out systems keep their dL, dE. */
+int socu_adjoint_scatter(const void* bbar, const void* x, void* dL, void* dE, const unsigned char* mask, int B, int N,
+                         int n, int m, int dtype, void* stream);
+
 /* Last error message of the calling thread ("" if none). */
 const char* socu_last_error(void);
 

@@ -14,5 +14,7 @@ from .block_tridiag_solver import (  # noqa: F401
     create_cholesky_solve_launch,
     optimal_problem_settings,
 )
+from .batched import create_batched_cholesky_launch  # noqa: F401
+from .mixed import create_cholesky_mixed_factor_and_solve_launch  # noqa: F401
 
 __version__ = "0.1.0"

@@ -5,6 +5,7 @@ The library lands next to this file so that it travels with the source tree.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -13,14 +14,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsocu_hip.so")
 
-SOURCES = ["nd_blocked.hip", "nd_refine.hip", "capi.hip"]
+SOURCES = ["nd_blocked.hip", "nd_refine.hip", "nd_batched.hip", "nd_adjoint.hip", "capi.hip"]
 
 HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "--offload-arch=gfx950",
     "-fPIC",
-    "-shared",
     # No packed-FP32 instructions anywhere in device code.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
@@ -32,16 +32,36 @@ def hipcc() -> str:
     return path if os.path.exists(path) else "hipcc"
 
 
-def build_library(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *srcs]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+def _run(cmd, what):
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed ({res.returncode}) building {OUT}")
+        raise RuntimeError(f"hipcc failed ({res.returncode}) {what}")
+
+
+def build_library(verbose: bool = False, jobs: int = 0) -> str:
+    """Compile every source to an object in parallel, then link the shared
+    library; the result replaces OUT atomically."""
+    objdir = os.path.join(HERE, "_objs")
+    os.makedirs(objdir, exist_ok=True)
+    jobs = jobs or min(len(SOURCES), os.cpu_count() or 1)
+    cmds = []
+    objs = []
+    for s in SOURCES:
+        obj = os.path.join(objdir, s.replace(".hip", ".o"))
+        objs.append(obj)
+        cmds.append(([hipcc(), *HIPCC_FLAGS, "-c", "-o", obj, os.path.join(CSRC, s)], f"compiling {s}"))
+    if verbose:
+        for cmd, _ in cmds:
+            print(" ".join(cmd), flush=True)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, c, w) for c, w in cmds]:
+            f.result()
+    tmp = OUT + ".tmp"
+    link = [hipcc(), "--offload-arch=gfx950", "-fPIC", "-shared", "-o", tmp, *objs]
+    if verbose:
+        print(" ".join(link), flush=True)
+    _run(link, f"linking {OUT}")
     os.replace(tmp, OUT)
     return OUT
 

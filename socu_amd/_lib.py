@@ -37,6 +37,13 @@ PREC_FP32_F32PIV = 1
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 
+
+class BatchSystem(ctypes.Structure):
+    """socu_batch_system: one independent system of a batched launch."""
+
+    _fields_ = [("L", _vp), ("E", _vp), ("x", _vp), ("N", _i), ("n", _i), ("m", _i), ("reserved", _i)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/socu_hip.h
 SIGNATURES = {
     "socu_calculate_recursive_iterations": (_i, [_i]),
@@ -57,6 +64,12 @@ SIGNATURES = {
     "socu_mixed_stats": (_i, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), _vp]),
     "socu_mixed_destroy": (_i, [_vp]),
     "socu_mixed_profile": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
+    "socu_batched_max_block_size": (_i, []),
+    "socu_batched_create": (_i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(BatchSystem), _i, _vp, _i, _i, _i]),
+    "socu_batched_execute": (_i, [_vp, _vp]),
+    "socu_batched_config": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "socu_batched_destroy": (_i, [_vp]),
+    "socu_adjoint_scatter": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "socu_last_error": (ctypes.c_char_p, []),
 }
 

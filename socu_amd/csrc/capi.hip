@@ -8,6 +8,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <stdlib.h>
 #include <string>
 #include <vector>
 
@@ -115,6 +119,7 @@ int ensure_init() {
   if (state == 1) return SOCU_OK;
   hipError_t e = socu::init_blocked_kernels();
   if (e == hipSuccess) e = socu::init_refine_kernels();
+  if (e == hipSuccess) e = socu::init_batched_kernels();
   if (e != hipSuccess) return hip_fail(e, "kernel set-up (hipFuncSetAttribute)");
   state = 1;
   return SOCU_OK;
@@ -213,6 +218,17 @@ int socu_cholesky_solve(const void* L, const void* E, void* x, const unsigned ch
 int socu_cholesky_factor_and_solve(void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
                                    int dtype, void* stream) {
   return dispatch(SOCU_OP_FACTOR_AND_SOLVE, L, E, x, mask, B, N, n, m, dtype, stream);
+}
+
+int socu_adjoint_scatter(const void* bbar, const void* x, void* dL, void* dE, const unsigned char* mask, int B, int N,
+                         int n, int m, int dtype, void* stream) {
+  if (bbar == nullptr || dL == nullptr || dE == nullptr) return fail(SOCU_EINVAL, "bbar, dL or dE is NULL");
+  int rc = check_dims(dL, dE, x, true, B, N, n, m, dtype);
+  if (rc) return rc;
+  if (B == 0) return SOCU_OK;
+  const hipError_t e = socu::adjoint_scatter(bbar, x, dL, dE, mask, B, N, n, m, dtype, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "adjoint scatter launch");
+  return SOCU_OK;
 }
 
 int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
@@ -326,8 +342,8 @@ int socu_mixed_create(socu_mixed** out, const void* L, const void* E, void* x, c
     return fail(SOCU_EINVAL, "unknown factor precision %d", precision);
   if (!(tol > 0.0)) return fail(SOCU_EINVAL, "tolerance must be positive");
   if (max_iter < 1 || max_iter > 100) return fail(SOCU_EINVAL, "max_iter must be in [1, 100] (got %d)", max_iter);
-  if (socu::mixed_residual_lds(n, m) > 160 * 1024)
-    return fail(SOCU_EUNSUPPORTED, "n*m = %d too large for the residual kernel (max 5120)", n * m);
+  if (!socu::mixed_fits(n, m))
+    return fail(SOCU_EUNSUPPORTED, "n*m = %d too large for the residual kernel (max 4608)", n * m);
   if (B > 0) {
     rc = ensure_init();
     if (rc) return rc;
@@ -439,6 +455,339 @@ int socu_mixed_destroy(socu_mixed* h) {
   if (h->graph) (void)hipGraphDestroy(h->graph);
   socu::mixed_free(h->p);
   delete h;
+  return SOCU_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched uneven systems.
+
+namespace {
+
+constexpr int kClasses = 3;
+const int kWaveChoices[3] = {1, 2, 4};
+
+// Process-wide tuning results: key -> waves per workgroup.  Optionally
+// persisted in the file named by SOCU_TUNE_CACHE ("key waves" per line).
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tune;
+bool g_tune_loaded = false;
+
+void tune_cache_load() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char* path = getenv("SOCU_TUNE_CACHE");
+  if (path == nullptr || *path == 0) return;
+  FILE* f = fopen(path, "r");
+  if (f == nullptr) return;
+  char key[256];
+  int w = 0;
+  while (fscanf(f, "%255s %d", key, &w) == 2)
+    if (w == 1 || w == 2 || w == 4) g_tune[key] = w;
+  fclose(f);
+}
+
+void tune_cache_store(const std::string& key, int w) {
+  g_tune[key] = w;
+  const char* path = getenv("SOCU_TUNE_CACHE");
+  if (path == nullptr || *path == 0) return;
+  FILE* f = fopen(path, "a");
+  if (f == nullptr) return;
+  fprintf(f, "%s %d\n", key.c_str(), w);
+  fclose(f);
+}
+
+}  // namespace
+
+struct socu_batched {
+  int op = 0, dtype = 0, count = 0;
+  const unsigned char* mask = nullptr;
+  socu::BatchProblem* dprobs = nullptr;  // device copy of the system table
+  int* dorder = nullptr;                 // device: system indices grouped by class
+  int counts[kClasses] = {0, 0, 0};
+  int starts[kClasses] = {0, 0, 0};
+  int waves[kClasses] = {1, 2, 4};
+  int launch_order[kClasses] = {2, 1, 0};  // classes by decreasing total cost
+  hipStream_t side[kClasses - 1] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kClasses - 1] = {nullptr, nullptr};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void batched_release(socu_batched* h) {
+  if (h == nullptr) return;
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  for (int k = 0; k < kClasses - 1; ++k) {
+    if (h->join[k]) (void)hipEventDestroy(h->join[k]);
+    if (h->side[k]) (void)hipStreamDestroy(h->side[k]);
+  }
+  if (h->fork) (void)hipEventDestroy(h->fork);
+  if (h->dprobs) (void)hipFree(h->dprobs);
+  if (h->dorder) (void)hipFree(h->dorder);
+  delete h;
+}
+
+// Launches every non-empty class: the costliest on st, the others on side
+// streams forked from and joined back into st, so their workgroups fill the
+// machine together.
+int batched_enqueue(socu_batched* h, const socu::BatchProblem* dprobs, const unsigned char* mask, hipStream_t st) {
+  int first = -1, nside = 0;
+  hipError_t e = hipSuccess;
+  for (int q = 0; q < kClasses && e == hipSuccess; ++q) {
+    const int c = h->launch_order[q];
+    if (h->counts[c] == 0) continue;
+    if (first < 0) {
+      first = c;
+      continue;
+    }
+    if (nside == 0) e = hipEventRecord(h->fork, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(h->side[nside], h->fork, 0);
+    if (e == hipSuccess)
+      e = socu::batched_launch_class(dprobs, h->dorder + h->starts[c], c, h->counts[c], h->waves[c], mask, h->op,
+                                     h->dtype, h->side[nside]);
+    if (e == hipSuccess) e = hipEventRecord(h->join[nside], h->side[nside]);
+    ++nside;
+  }
+  if (e == hipSuccess && first >= 0)
+    e = socu::batched_launch_class(dprobs, h->dorder + h->starts[first], first, h->counts[first], h->waves[first],
+                                   mask, h->op, h->dtype, st);
+  for (int k = 0; k < nside && e == hipSuccess; ++k) e = hipStreamWaitEvent(st, h->join[k], 0);
+  if (e != hipSuccess) return hip_fail(e, "batched launch");
+  return SOCU_OK;
+}
+
+// Times each wave count per class on a device copy of the batch and keeps
+// the fastest.  The caller's arrays are only read.
+int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs, const long long* class_cost) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  tune_cache_load();
+  std::string keys[kClasses];
+  bool need = false;
+  for (int c = 0; c < kClasses; ++c) {
+    if (h->counts[c] == 0) continue;
+    char buf[128];
+    snprintf(buf, sizeof buf, "batched:%s:op%d:nt%d:count%d:cost%lld", h->dtype == SOCU_DTYPE_F64 ? "f64" : "f32",
+             h->op, 8 << c, h->counts[c], class_cost[c]);
+    keys[c] = buf;
+    auto it = g_tune.find(keys[c]);
+    if (it != g_tune.end()) h->waves[c] = it->second;
+    else need = true;
+  }
+  if (!need) return SOCU_OK;
+  const size_t es = h->dtype == SOCU_DTYPE_F64 ? sizeof(double) : sizeof(float);
+  std::vector<socu::BatchProblem> copy(probs);
+  size_t total = 0;
+  std::vector<size_t> offs(probs.size());
+  for (size_t k = 0; k < probs.size(); ++k) {
+    const auto& P = probs[k];
+    const size_t nn = (size_t)P.n * P.n;
+    offs[k] = total;
+    total += ((size_t)P.N * nn + (size_t)socu::off_diag_storage_len(P.N) * nn + (size_t)P.N * P.n * P.m) * es;
+    total = (total + 255) & ~(size_t)255;
+  }
+  char* arena = nullptr;
+  socu::BatchProblem* dcopy = nullptr;
+  hipStream_t ts = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = SOCU_OK;
+  hipError_t e = hipMalloc(&arena, total);
+  if (e == hipSuccess) {
+    for (size_t k = 0; k < probs.size(); ++k) {
+      const auto& P = probs[k];
+      const size_t nn = (size_t)P.n * P.n;
+      char* b = arena + offs[k];
+      copy[k].L = b;
+      copy[k].E = b + (size_t)P.N * nn * es;
+      copy[k].x = (P.x != nullptr) ? b + ((size_t)P.N * nn + (size_t)socu::off_diag_storage_len(P.N) * nn) * es
+                                   : nullptr;
+    }
+    e = hipMalloc(&dcopy, copy.size() * sizeof(socu::BatchProblem));
+  }
+  if (e == hipSuccess) e = hipMemcpy(dcopy, copy.data(), copy.size() * sizeof(socu::BatchProblem), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ts, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  for (int c = 0; c < kClasses && e == hipSuccess; ++c) {
+    if (h->counts[c] == 0 || g_tune.count(keys[c])) continue;
+    float best = 1e30f;
+    int best_w = h->waves[c];
+    for (int wi = 0; wi < 3 && e == hipSuccess; ++wi) {
+      const int w = kWaveChoices[wi];
+      for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+        e = socu::batched_copy(h->dprobs, dcopy, h->count, h->op, h->dtype, ts);
+        if (e == hipSuccess) e = hipEventRecord(e0, ts);
+        if (e == hipSuccess)
+          e = socu::batched_launch_class(dcopy, h->dorder + h->starts[c], c, h->counts[c], w, nullptr, h->op,
+                                         h->dtype, ts);
+        if (e == hipSuccess) e = hipEventRecord(e1, ts);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && ms < best) {
+          best = ms;
+          best_w = w;
+        }
+      }
+    }
+    if (e == hipSuccess) {
+      h->waves[c] = best_w;
+      tune_cache_store(keys[c], best_w);
+    }
+  }
+  if (e != hipSuccess) rc = hip_fail(e, "batched autotune");
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (ts) (void)hipStreamDestroy(ts);
+  if (dcopy) (void)hipFree(dcopy);
+  if (arena) (void)hipFree(arena);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int socu_batched_max_block_size(void) { return 32; }
+
+int socu_batched_create(socu_batched** out, int op, const socu_batch_system* systems, int count,
+                        const unsigned char* mask, int dtype, int tune, int use_graph) {
+  if (out == nullptr) return fail(SOCU_EINVAL, "batched output pointer is NULL");
+  *out = nullptr;
+  if (op < SOCU_OP_FACTOR || op > SOCU_OP_BACKWARD) return fail(SOCU_EINVAL, "unknown operation %d", op);
+  if (dtype != SOCU_DTYPE_F32 && dtype != SOCU_DTYPE_F64) return fail(SOCU_EINVAL, "unknown dtype code %d", dtype);
+  if (count < 0) return fail(SOCU_EINVAL, "system count must be >= 0, got %d", count);
+  if (count > 0 && systems == nullptr) return fail(SOCU_EINVAL, "system table is NULL");
+  const bool need_x = op != SOCU_OP_FACTOR;
+  std::vector<socu::BatchProblem> probs(count);
+  std::vector<long long> cost(count);
+  std::vector<int> cls(count);
+  for (int k = 0; k < count; ++k) {
+    const socu_batch_system& S = systems[k];
+    if (S.L == nullptr || (S.E == nullptr && S.N > 1)) return fail(SOCU_EINVAL, "system %d: L or E is NULL", k);
+    if (S.N < 1) return fail(SOCU_EINVAL, "system %d: horizon N must be >= 1, got %d", k, S.N);
+    if (S.n < 1) return fail(SOCU_EINVAL, "system %d: block size n must be >= 1, got %d", k, S.n);
+    if (S.reserved != 0) return fail(SOCU_EINVAL, "system %d: reserved field must be 0", k);
+    const int m = need_x ? S.m : 1;
+    if (need_x && (S.x == nullptr || m < 1)) return fail(SOCU_EINVAL, "system %d: x is NULL or m < 1", k);
+    if (S.n > 32 || m > 32)
+      return fail(SOCU_EUNSUPPORTED, "system %d: batched systems need n <= 32 and m <= 32 (got n=%d, m=%d)", k, S.n,
+                  m);
+    probs[k] = socu::BatchProblem{S.L, S.E, need_x ? S.x : nullptr, S.N, S.n, m, 0};
+    cost[k] = (long long)S.N * S.n * S.n * (S.n + m);
+    cls[k] = socu::batched_class(std::max(S.n, m));
+  }
+  socu_batched* h = new socu_batched();
+  h->op = op;
+  h->dtype = dtype;
+  h->count = count;
+  h->mask = mask;
+  if (count == 0) {
+    *out = h;
+    return SOCU_OK;
+  }
+  int rc = ensure_init();
+  if (rc) {
+    batched_release(h);
+    return rc;
+  }
+  // Order: grouped by class, each group by decreasing cost (longest systems
+  // are dispatched first so the tail is made of short ones).
+  std::vector<int> order(count);
+  for (int k = 0; k < count; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (cls[a] != cls[b]) return cls[a] > cls[b];
+    return cost[a] > cost[b];
+  });
+  long long class_cost[kClasses] = {0, 0, 0};
+  for (int k = 0; k < count; ++k) {
+    h->counts[cls[k]] += 1;
+    class_cost[cls[k]] += cost[k];
+  }
+  h->starts[2] = 0;
+  h->starts[1] = h->counts[2];
+  h->starts[0] = h->counts[2] + h->counts[1];
+  std::sort(h->launch_order, h->launch_order + kClasses, [&](int a, int b) { return class_cost[a] > class_cost[b]; });
+  hipError_t e = hipMalloc(&h->dprobs, count * sizeof(socu::BatchProblem));
+  if (e == hipSuccess) e = hipMalloc(&h->dorder, count * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(h->dprobs, probs.data(), count * sizeof(socu::BatchProblem), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(h->dorder, order.data(), count * sizeof(int), hipMemcpyHostToDevice);
+  for (int k = 0; k < kClasses - 1 && e == hipSuccess; ++k) {
+    e = hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->join[k], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->fork, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    batched_release(h);
+    return hip_fail(e, "batched plan set-up");
+  }
+  if (tune) {
+    rc = batched_tune(h, probs, class_cost);
+    if (rc) {
+      batched_release(h);
+      return rc;
+    }
+  }
+  if (use_graph) {
+    hipStream_t cs;
+    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      batched_release(h);
+      return hip_fail(e, "hipStreamCreate (graph capture)");
+    }
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      (void)hipStreamDestroy(cs);
+      batched_release(h);
+      return hip_fail(e, "hipStreamBeginCapture");
+    }
+    rc = batched_enqueue(h, h->dprobs, h->mask, cs);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (rc != SOCU_OK || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      batched_release(h);
+      return rc != SOCU_OK ? rc : hip_fail(e2, "hipStreamEndCapture");
+    }
+    h->graph = g;
+    e = hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      batched_release(h);
+      return hip_fail(e, "hipGraphInstantiate");
+    }
+  }
+  *out = h;
+  return SOCU_OK;
+}
+
+int socu_batched_execute(socu_batched* h, void* stream) {
+  if (h == nullptr) return fail(SOCU_EINVAL, "batched plan is NULL");
+  if (h->count == 0) return SOCU_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (h->exec) {
+    hipError_t e = hipGraphLaunch(h->exec, st);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    return SOCU_OK;
+  }
+  return batched_enqueue(h, h->dprobs, h->mask, st);
+}
+
+int socu_batched_config(socu_batched* h, int* counts, int* waves) {
+  if (h == nullptr || counts == nullptr || waves == nullptr) return fail(SOCU_EINVAL, "NULL argument");
+  for (int c = 0; c < kClasses; ++c) {
+    counts[c] = h->counts[c];
+    waves[c] = h->waves[c];
+  }
+  return SOCU_OK;
+}
+
+int socu_batched_destroy(socu_batched* h) {
+  batched_release(h);
   return SOCU_OK;
 }
 

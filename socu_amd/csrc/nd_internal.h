@@ -66,9 +66,34 @@ struct MixedPlan {
   int nblk_init = 1;
 };
 hipError_t init_refine_kernels();
+
+// Batched systems of uneven size (nd_batched.hip): one entry per system, all
+// arrays contiguous row-major in the system's own N and n (n <= 32, m <= 32).
+struct BatchProblem {
+  void* L;  // (N, n, n)
+  void* E;  // (n_E(N), n, n)
+  void* x;  // (N, n, m), may be NULL for a factor
+  int N, n, m, pad;
+};
+hipError_t init_batched_kernels();
+// Tile class of a block size: 0 (tiles of 8), 1 (16), 2 (32).
+int batched_class(int n);
+// One launch over `count` systems of tile class `cls`, listed (by index into
+// dprobs) at dorder, with workgroups of `waves` (1, 2 or 4) wavefronts.
+hipError_t batched_launch_class(const BatchProblem* dprobs, const int* dorder, int cls, int count, int waves,
+                                const uint8_t* mask, int op, int dtype, hipStream_t st);
+// Copies L, E (and x unless op is a factor) of count systems src[k] -> dst[k].
+hipError_t batched_copy(const BatchProblem* dsrc, const BatchProblem* ddst, int count, int op, int dtype,
+                        hipStream_t st);
 size_t mixed_residual_lds(int n, int m);
+bool mixed_fits(int n, int m);
 hipError_t mixed_alloc(MixedPlan& p);
 void mixed_free(MixedPlan& p);
 hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st);
+
+// Adjoint scatter (nd_adjoint.hip): dL, dE of x = Psi^{-1} b from
+// bbar = Psi^{-1} xbar and x; all arrays in the solver's (B, ...) layout.
+hipError_t adjoint_scatter(const void* bbar, const void* x, void* dL, void* dE, const uint8_t* mask, int B, int N,
+                           int n, int m, int dtype, hipStream_t st);
 
 }  // namespace socu

@@ -228,9 +228,16 @@ static int refine_blocks(size_t count) {
 
 size_t mixed_residual_lds(int n, int m) { return sizeof(double) * 4 * (size_t)n * m; }
 
+// Dynamic LDS cap of the residual kernel (its static LDS is 2 KiB of the
+// 160 KiB per workgroup).
+constexpr size_t kResidualLdsMax = 144 * 1024;
+
 hipError_t init_refine_kernels() {
-  return hipFuncSetAttribute((const void*)k_residual, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  return hipFuncSetAttribute((const void*)k_residual, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)kResidualLdsMax);
 }
+
+bool mixed_fits(int n, int m) { return mixed_residual_lds(n, m) <= kResidualLdsMax; }
 
 hipError_t mixed_alloc(MixedPlan& p) {
   const size_t nn = (size_t)p.n * p.n, nm = (size_t)p.n * p.m;

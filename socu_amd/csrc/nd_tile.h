@@ -179,4 +179,50 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
   if (rowlane) dinv[r] = rinv;
 }
 
+// Cholesky and inverse of G = 64/(2*NT) independent NT x NT tiles per
+// wavefront (NT = 8, 16 or 32): lanes [g*2NT, g*2NT+NT) keep the rows of tile
+// g, lanes [g*2NT+NT, (g+1)*2NT) its inverse columns, with the same recurrence
+// as tile_chol_inv.  Tile g is S0 + g*stride (row-major, pitch `pitch`), its
+// NT-entry column buffer col0 + g*stride and its 1/L[r][r] output
+// dinv0 + g*stride.  The pivot comes through the column buffer, since each
+// group has its own source lane.
+template <typename T, int NT>
+__device__ __forceinline__ void tile_chol_inv_grouped(T* S0, T* col0, T* dinv0, int stride, int pitch) {
+  const int lane = lane_id();
+  const int g = lane / (2 * NT);
+  const int lr = lane % (2 * NT);
+  const bool rowlane = lr < NT;
+  const int r = lr % NT;
+  T* St = S0 + g * stride;
+  T* cb = col0 + g * stride;
+  T u[NT];
+  T rinv = T(1);
+#pragma unroll
+  for (int c = 0; c < NT; ++c) {
+    const T v = St[r * pitch + c];
+    u[c] = rowlane ? ((c <= r) ? v : T(0)) : ((c == r) ? T(1) : T(0));
+  }
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    if (rowlane) cb[r] = u[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const T p = cb[k];
+    const T inv = rsqrt_t(p);
+    const T lk = rowlane ? ((r > k) ? u[k] * inv : T(0)) : u[k] * inv;
+    const T coef = lk * inv;
+#pragma unroll
+    for (int q = k + 1; q < NT; ++q) u[q] -= coef * cb[q];
+    u[k] = rowlane ? ((r == k) ? p * inv : ((r > k) ? lk : u[k])) : lk;
+    rinv = (r == k) ? inv : rinv;
+#pragma unroll
+    for (int q = 0; q < NT; ++q) asm volatile("" : "+v"(u[q]));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  }
+#pragma unroll
+  for (int c = 0; c < NT; ++c)
+    if (rowlane ? (c <= r) : (c > r)) St[r * pitch + c] = u[c];
+  if (rowlane) dinv0[g * stride + r] = rinv;
+}
+
 }  // namespace socu
