@@ -17,8 +17,9 @@
  *    batch entry's L, E and x untouched (README.md:214-235).  It is read by
  *    the kernels on every execution, including graph replays.
  *  - stream is a hipStream_t (NULL = the null stream).  Work is enqueued
- *    asynchronously; nothing synchronises the host.  No entry point allocates
- *    device memory.
+ *    asynchronously; nothing synchronises the host.  Only the *_create
+ *    calls allocate device memory (plan workspaces, graphs); the direct
+ *    entry points and every *_execute allocate nothing.
  *  - Return 0 on success, a negative SOCU_E* code otherwise;
  *    socu_last_error() returns the calling thread's last message.
  */
@@ -130,15 +131,16 @@ int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask
  * (N,n,m) (ignored for SOCU_OP_FACTOR); `reserved` must be 0.  Needs
  * n <= socu_batched_max_block_size() and m <= 32.  mask is NULL or `count`
  * bytes indexed like `systems`.  One workgroup per system walks all its
- * levels; systems are grouped in tile classes (8, 16, 32 by max(n, m)), one
- * launch per class, the classes on forked streams.  With `tune` the create
- * call times each workgroup size (1, 2, 4 wavefronts) per class on a device
- * copy of the batch (the caller's arrays are only read) and keeps the
- * fastest; results are cached per process and, if the environment variable
- * SOCU_TUNE_CACHE names a file, appended there and reused by later
- * processes.  create copies the system table to the device and synchronises;
- * execute allocates nothing and synchronises nothing.
- * socu_batched_config returns per class the system count and wave count. */
+ * levels.  Systems fall in size buckets by max(n, m) (<= 8, <= 16, <= 32);
+ * each bucket runs with a tile size of at least its bound, one launch per
+ * tile size, the launches on forked streams.  With `tune` the create call
+ * times every admissible tile size per bucket on a device copy of the batch
+ * (the caller's arrays are only read) and keeps the fastest; results are
+ * cached per process and, if the environment variable SOCU_TUNE_CACHE names
+ * a file, appended there and reused by later processes.  Without `tune` each
+ * bucket uses its own bound.  create copies the system table to the device
+ * and synchronises; execute allocates nothing and synchronises nothing.
+ * socu_batched_config returns per bucket the system count and tile size. */
 typedef struct socu_batch_system {
   void* L;
   void* E;
@@ -153,7 +155,7 @@ int socu_batched_max_block_size(void);
 int socu_batched_create(socu_batched** out, int op, const socu_batch_system* systems, int count,
                         const unsigned char* mask, int dtype, int tune, int use_graph);
 int socu_batched_execute(socu_batched* handle, void* stream);
-int socu_batched_config(socu_batched* handle, int* counts, int* waves);
+int socu_batched_config(socu_batched* handle, int* counts, int* tiles);
 int socu_batched_destroy(socu_batched* handle);
 
 /* Adjoint scatter (no counterpart in the reference, which has no gradient).

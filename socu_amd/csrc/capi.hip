@@ -222,7 +222,8 @@ int socu_cholesky_factor_and_solve(void* L, void* E, void* x, const unsigned cha
 
 int socu_adjoint_scatter(const void* bbar, const void* x, void* dL, void* dE, const unsigned char* mask, int B, int N,
                          int n, int m, int dtype, void* stream) {
-  if (bbar == nullptr || dL == nullptr || dE == nullptr) return fail(SOCU_EINVAL, "bbar, dL or dE is NULL");
+  if (bbar == nullptr || dL == nullptr || (dE == nullptr && N > 1))
+    return fail(SOCU_EINVAL, "bbar, dL or dE is NULL");
   int rc = check_dims(dL, dE, x, true, B, N, n, m, dtype);
   if (rc) return rc;
   if (B == 0) return SOCU_OK;
@@ -465,11 +466,10 @@ int socu_mixed_destroy(socu_mixed* h) {
 
 namespace {
 
-constexpr int kClasses = 3;
-const int kWaveChoices[3] = {1, 2, 4};
+constexpr int kClasses = 3;  // tile classes NT = 8, 16, 32
 
-// Process-wide tuning results: key -> waves per workgroup.  Optionally
-// persisted in the file named by SOCU_TUNE_CACHE ("key waves" per line).
+// Process-wide tuning results: key -> chosen tile class.  Optionally
+// persisted in the file named by SOCU_TUNE_CACHE ("key class" per line).
 std::mutex g_tune_mu;
 std::map<std::string, int> g_tune;
 bool g_tune_loaded = false;
@@ -482,33 +482,42 @@ void tune_cache_load() {
   FILE* f = fopen(path, "r");
   if (f == nullptr) return;
   char key[256];
-  int w = 0;
-  while (fscanf(f, "%255s %d", key, &w) == 2)
-    if (w == 1 || w == 2 || w == 4) g_tune[key] = w;
+  int c = 0;
+  while (fscanf(f, "%255s %d", key, &c) == 2)
+    if (c >= 0 && c < kClasses) g_tune[key] = c;
   fclose(f);
 }
 
-void tune_cache_store(const std::string& key, int w) {
-  g_tune[key] = w;
+void tune_cache_store(const std::string& key, int c) {
+  g_tune[key] = c;
   const char* path = getenv("SOCU_TUNE_CACHE");
   if (path == nullptr || *path == 0) return;
   FILE* f = fopen(path, "a");
   if (f == nullptr) return;
-  fprintf(f, "%s %d\n", key.c_str(), w);
+  fprintf(f, "%s %d\n", key.c_str(), c);
   fclose(f);
 }
 
 }  // namespace
 
+// Systems are grouped in size buckets b (max(n, m) <= 8, 16, 32); bucket b
+// runs with tile class tile[b] >= b.  Systems of buckets that share a tile
+// class form one launch.
 struct socu_batched {
   int op = 0, dtype = 0, count = 0;
   const unsigned char* mask = nullptr;
   socu::BatchProblem* dprobs = nullptr;  // device copy of the system table
-  int* dorder = nullptr;                 // device: system indices grouped by class
-  int counts[kClasses] = {0, 0, 0};
+  int* dorder = nullptr;                 // device: system indices grouped by launch class
+  std::vector<int> order;                // host copy of dorder
+  std::vector<int> bucket_of;            // per system
+  std::vector<long long> cost;           // per system
+  std::vector<int> mrhs;                 // per system
+  int bucket_count[kClasses] = {0, 0, 0};
+  int tile[kClasses] = {0, 1, 2};
+  int counts[kClasses] = {0, 0, 0};  // per launch class
   int starts[kClasses] = {0, 0, 0};
-  int waves[kClasses] = {1, 2, 4};
-  int launch_order[kClasses] = {2, 1, 0};  // classes by decreasing total cost
+  int mmax[kClasses] = {1, 1, 1};
+  int launch_order[kClasses] = {2, 1, 0};  // launch classes by decreasing total cost
   hipStream_t side[kClasses - 1] = {nullptr, nullptr};
   hipEvent_t fork = nullptr;
   hipEvent_t join[kClasses - 1] = {nullptr, nullptr};
@@ -532,10 +541,44 @@ void batched_release(socu_batched* h) {
   delete h;
 }
 
+// Host order of the systems of `buckets` (a bit set) by decreasing cost.
+std::vector<int> bucket_order(const socu_batched* h, unsigned buckets) {
+  std::vector<int> o;
+  for (int k = 0; k < h->count; ++k)
+    if (buckets & (1u << h->bucket_of[k])) o.push_back(k);
+  std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return h->cost[a] > h->cost[b]; });
+  return o;
+}
+
+// Builds the launch classes from h->tile and uploads the order.
+int batched_layout(socu_batched* h) {
+  h->order.clear();
+  long long ccost[kClasses] = {0, 0, 0};
+  for (int c = kClasses - 1; c >= 0; --c) {
+    unsigned set = 0;
+    for (int b = 0; b < kClasses; ++b)
+      if (h->tile[b] == c) set |= 1u << b;
+    const std::vector<int> o = bucket_order(h, set);
+    h->starts[c] = (int)h->order.size();
+    h->counts[c] = (int)o.size();
+    h->mmax[c] = 1;
+    for (int k : o) {
+      h->mmax[c] = std::max(h->mmax[c], h->mrhs[k]);
+      ccost[c] += h->cost[k];
+    }
+    h->order.insert(h->order.end(), o.begin(), o.end());
+  }
+  for (int c = 0; c < kClasses; ++c) h->launch_order[c] = c;
+  std::sort(h->launch_order, h->launch_order + kClasses, [&](int a, int b) { return ccost[a] > ccost[b]; });
+  const hipError_t e = hipMemcpy(h->dorder, h->order.data(), h->order.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "batched plan set-up (order)");
+  return SOCU_OK;
+}
+
 // Launches every non-empty class: the costliest on st, the others on side
 // streams forked from and joined back into st, so their workgroups fill the
 // machine together.
-int batched_enqueue(socu_batched* h, const socu::BatchProblem* dprobs, const unsigned char* mask, hipStream_t st) {
+int batched_enqueue(socu_batched* h, const unsigned char* mask, hipStream_t st) {
   int first = -1, nside = 0;
   hipError_t e = hipSuccess;
   for (int q = 0; q < kClasses && e == hipSuccess; ++q) {
@@ -548,34 +591,38 @@ int batched_enqueue(socu_batched* h, const socu::BatchProblem* dprobs, const uns
     if (nside == 0) e = hipEventRecord(h->fork, st);
     if (e == hipSuccess) e = hipStreamWaitEvent(h->side[nside], h->fork, 0);
     if (e == hipSuccess)
-      e = socu::batched_launch_class(dprobs, h->dorder + h->starts[c], c, h->counts[c], h->waves[c], mask, h->op,
+      e = socu::batched_launch_class(h->dprobs, h->dorder + h->starts[c], c, h->counts[c], h->mmax[c], mask, h->op,
                                      h->dtype, h->side[nside]);
     if (e == hipSuccess) e = hipEventRecord(h->join[nside], h->side[nside]);
     ++nside;
   }
   if (e == hipSuccess && first >= 0)
-    e = socu::batched_launch_class(dprobs, h->dorder + h->starts[first], first, h->counts[first], h->waves[first],
+    e = socu::batched_launch_class(h->dprobs, h->dorder + h->starts[first], first, h->counts[first], h->mmax[first],
                                    mask, h->op, h->dtype, st);
   for (int k = 0; k < nside && e == hipSuccess; ++k) e = hipStreamWaitEvent(st, h->join[k], 0);
   if (e != hipSuccess) return hip_fail(e, "batched launch");
   return SOCU_OK;
 }
 
-// Times each wave count per class on a device copy of the batch and keeps
-// the fastest.  The caller's arrays are only read.
-int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs, const long long* class_cost) {
+// For each size bucket, times every admissible tile class (>= the bucket's
+// own) on a device copy of the batch and keeps the fastest.  The caller's
+// arrays are only read.
+int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   tune_cache_load();
   std::string keys[kClasses];
   bool need = false;
-  for (int c = 0; c < kClasses; ++c) {
-    if (h->counts[c] == 0) continue;
-    char buf[128];
-    snprintf(buf, sizeof buf, "batched:%s:op%d:nt%d:count%d:cost%lld", h->dtype == SOCU_DTYPE_F64 ? "f64" : "f32",
-             h->op, 8 << c, h->counts[c], class_cost[c]);
-    keys[c] = buf;
-    auto it = g_tune.find(keys[c]);
-    if (it != g_tune.end()) h->waves[c] = it->second;
+  for (int b = 0; b < kClasses; ++b) {
+    if (h->bucket_count[b] == 0) continue;
+    long long bc = 0;
+    for (int k = 0; k < h->count; ++k)
+      if (h->bucket_of[k] == b) bc += h->cost[k];
+    char buf[160];
+    snprintf(buf, sizeof buf, "batched2:%s:op%d:bucket%d:count%d:cost%lld", h->dtype == SOCU_DTYPE_F64 ? "f64" : "f32",
+             h->op, b, h->bucket_count[b], bc);
+    keys[b] = buf;
+    auto it = g_tune.find(keys[b]);
+    if (it != g_tune.end() && it->second >= b) h->tile[b] = it->second;
     else need = true;
   }
   if (!need) return SOCU_OK;
@@ -592,6 +639,7 @@ int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs, 
   }
   char* arena = nullptr;
   socu::BatchProblem* dcopy = nullptr;
+  int* dord = nullptr;
   hipStream_t ts = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = SOCU_OK;
@@ -600,49 +648,53 @@ int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs, 
     for (size_t k = 0; k < probs.size(); ++k) {
       const auto& P = probs[k];
       const size_t nn = (size_t)P.n * P.n;
-      char* b = arena + offs[k];
-      copy[k].L = b;
-      copy[k].E = b + (size_t)P.N * nn * es;
-      copy[k].x = (P.x != nullptr) ? b + ((size_t)P.N * nn + (size_t)socu::off_diag_storage_len(P.N) * nn) * es
+      char* base = arena + offs[k];
+      copy[k].L = base;
+      copy[k].E = base + (size_t)P.N * nn * es;
+      copy[k].x = (P.x != nullptr) ? base + ((size_t)P.N * nn + (size_t)socu::off_diag_storage_len(P.N) * nn) * es
                                    : nullptr;
     }
     e = hipMalloc(&dcopy, copy.size() * sizeof(socu::BatchProblem));
   }
+  if (e == hipSuccess) e = hipMalloc(&dord, h->count * sizeof(int));
   if (e == hipSuccess) e = hipMemcpy(dcopy, copy.data(), copy.size() * sizeof(socu::BatchProblem), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ts, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  for (int c = 0; c < kClasses && e == hipSuccess; ++c) {
-    if (h->counts[c] == 0 || g_tune.count(keys[c])) continue;
+  for (int b = 0; b < kClasses && e == hipSuccess; ++b) {
+    if (h->bucket_count[b] == 0 || (g_tune.count(keys[b]) && g_tune[keys[b]] >= b)) continue;
+    const std::vector<int> o = bucket_order(h, 1u << b);
+    int mm = 1;
+    for (int k : o) mm = std::max(mm, h->mrhs[k]);
+    e = hipMemcpy(dord, o.data(), o.size() * sizeof(int), hipMemcpyHostToDevice);
     float best = 1e30f;
-    int best_w = h->waves[c];
-    for (int wi = 0; wi < 3 && e == hipSuccess; ++wi) {
-      const int w = kWaveChoices[wi];
+    int best_c = b;
+    for (int c = b; c < kClasses && e == hipSuccess; ++c) {
       for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
         e = socu::batched_copy(h->dprobs, dcopy, h->count, h->op, h->dtype, ts);
         if (e == hipSuccess) e = hipEventRecord(e0, ts);
         if (e == hipSuccess)
-          e = socu::batched_launch_class(dcopy, h->dorder + h->starts[c], c, h->counts[c], w, nullptr, h->op,
-                                         h->dtype, ts);
+          e = socu::batched_launch_class(dcopy, dord, c, (int)o.size(), mm, nullptr, h->op, h->dtype, ts);
         if (e == hipSuccess) e = hipEventRecord(e1, ts);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
         if (e == hipSuccess && ms < best) {
           best = ms;
-          best_w = w;
+          best_c = c;
         }
       }
     }
     if (e == hipSuccess) {
-      h->waves[c] = best_w;
-      tune_cache_store(keys[c], best_w);
+      h->tile[b] = best_c;
+      tune_cache_store(keys[b], best_c);
     }
   }
   if (e != hipSuccess) rc = hip_fail(e, "batched autotune");
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (ts) (void)hipStreamDestroy(ts);
+  if (dord) (void)hipFree(dord);
   if (dcopy) (void)hipFree(dcopy);
   if (arena) (void)hipFree(arena);
   return rc;
@@ -664,28 +716,35 @@ int socu_batched_create(socu_batched** out, int op, const socu_batch_system* sys
   if (count > 0 && systems == nullptr) return fail(SOCU_EINVAL, "system table is NULL");
   const bool need_x = op != SOCU_OP_FACTOR;
   std::vector<socu::BatchProblem> probs(count);
-  std::vector<long long> cost(count);
-  std::vector<int> cls(count);
-  for (int k = 0; k < count; ++k) {
-    const socu_batch_system& S = systems[k];
-    if (S.L == nullptr || (S.E == nullptr && S.N > 1)) return fail(SOCU_EINVAL, "system %d: L or E is NULL", k);
-    if (S.N < 1) return fail(SOCU_EINVAL, "system %d: horizon N must be >= 1, got %d", k, S.N);
-    if (S.n < 1) return fail(SOCU_EINVAL, "system %d: block size n must be >= 1, got %d", k, S.n);
-    if (S.reserved != 0) return fail(SOCU_EINVAL, "system %d: reserved field must be 0", k);
-    const int m = need_x ? S.m : 1;
-    if (need_x && (S.x == nullptr || m < 1)) return fail(SOCU_EINVAL, "system %d: x is NULL or m < 1", k);
-    if (S.n > 32 || m > 32)
-      return fail(SOCU_EUNSUPPORTED, "system %d: batched systems need n <= 32 and m <= 32 (got n=%d, m=%d)", k, S.n,
-                  m);
-    probs[k] = socu::BatchProblem{S.L, S.E, need_x ? S.x : nullptr, S.N, S.n, m, 0};
-    cost[k] = (long long)S.N * S.n * S.n * (S.n + m);
-    cls[k] = socu::batched_class(std::max(S.n, m));
-  }
   socu_batched* h = new socu_batched();
   h->op = op;
   h->dtype = dtype;
   h->count = count;
   h->mask = mask;
+  h->bucket_of.resize(count);
+  h->cost.resize(count);
+  h->mrhs.resize(count);
+  for (int k = 0; k < count; ++k) {
+    const socu_batch_system& S = systems[k];
+    int rc = SOCU_OK;
+    const int m = need_x ? S.m : 1;
+    if (S.L == nullptr || (S.E == nullptr && S.N > 1)) rc = fail(SOCU_EINVAL, "system %d: L or E is NULL", k);
+    else if (S.N < 1) rc = fail(SOCU_EINVAL, "system %d: horizon N must be >= 1, got %d", k, S.N);
+    else if (S.n < 1) rc = fail(SOCU_EINVAL, "system %d: block size n must be >= 1, got %d", k, S.n);
+    else if (S.reserved != 0) rc = fail(SOCU_EINVAL, "system %d: reserved field must be 0", k);
+    else if (need_x && (S.x == nullptr || m < 1)) rc = fail(SOCU_EINVAL, "system %d: x is NULL or m < 1", k);
+    else if (S.n > 32 || m > 32)
+      rc = fail(SOCU_EUNSUPPORTED, "system %d: batched systems need n <= 32 and m <= 32 (got n=%d, m=%d)", k, S.n, m);
+    if (rc) {
+      delete h;
+      return rc;
+    }
+    probs[k] = socu::BatchProblem{S.L, S.E, need_x ? S.x : nullptr, S.N, S.n, m, 0};
+    h->cost[k] = (long long)S.N * S.n * S.n * (S.n + m);
+    h->bucket_of[k] = socu::batched_class(std::max(S.n, m));
+    h->mrhs[k] = m;
+    h->bucket_count[h->bucket_of[k]] += 1;
+  }
   if (count == 0) {
     *out = h;
     return SOCU_OK;
@@ -695,27 +754,9 @@ int socu_batched_create(socu_batched** out, int op, const socu_batch_system* sys
     batched_release(h);
     return rc;
   }
-  // Order: grouped by class, each group by decreasing cost (longest systems
-  // are dispatched first so the tail is made of short ones).
-  std::vector<int> order(count);
-  for (int k = 0; k < count; ++k) order[k] = k;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    if (cls[a] != cls[b]) return cls[a] > cls[b];
-    return cost[a] > cost[b];
-  });
-  long long class_cost[kClasses] = {0, 0, 0};
-  for (int k = 0; k < count; ++k) {
-    h->counts[cls[k]] += 1;
-    class_cost[cls[k]] += cost[k];
-  }
-  h->starts[2] = 0;
-  h->starts[1] = h->counts[2];
-  h->starts[0] = h->counts[2] + h->counts[1];
-  std::sort(h->launch_order, h->launch_order + kClasses, [&](int a, int b) { return class_cost[a] > class_cost[b]; });
   hipError_t e = hipMalloc(&h->dprobs, count * sizeof(socu::BatchProblem));
   if (e == hipSuccess) e = hipMalloc(&h->dorder, count * sizeof(int));
   if (e == hipSuccess) e = hipMemcpy(h->dprobs, probs.data(), count * sizeof(socu::BatchProblem), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(h->dorder, order.data(), count * sizeof(int), hipMemcpyHostToDevice);
   for (int k = 0; k < kClasses - 1 && e == hipSuccess; ++k) {
     e = hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->join[k], hipEventDisableTiming);
@@ -725,12 +766,11 @@ int socu_batched_create(socu_batched** out, int op, const socu_batch_system* sys
     batched_release(h);
     return hip_fail(e, "batched plan set-up");
   }
-  if (tune) {
-    rc = batched_tune(h, probs, class_cost);
-    if (rc) {
-      batched_release(h);
-      return rc;
-    }
+  if (tune) rc = batched_tune(h, probs);
+  if (rc == SOCU_OK) rc = batched_layout(h);
+  if (rc) {
+    batched_release(h);
+    return rc;
   }
   if (use_graph) {
     hipStream_t cs;
@@ -745,7 +785,7 @@ int socu_batched_create(socu_batched** out, int op, const socu_batch_system* sys
       batched_release(h);
       return hip_fail(e, "hipStreamBeginCapture");
     }
-    rc = batched_enqueue(h, h->dprobs, h->mask, cs);
+    rc = batched_enqueue(h, h->mask, cs);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
@@ -774,14 +814,14 @@ int socu_batched_execute(socu_batched* h, void* stream) {
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
     return SOCU_OK;
   }
-  return batched_enqueue(h, h->dprobs, h->mask, st);
+  return batched_enqueue(h, h->mask, st);
 }
 
-int socu_batched_config(socu_batched* h, int* counts, int* waves) {
-  if (h == nullptr || counts == nullptr || waves == nullptr) return fail(SOCU_EINVAL, "NULL argument");
-  for (int c = 0; c < kClasses; ++c) {
-    counts[c] = h->counts[c];
-    waves[c] = h->waves[c];
+int socu_batched_config(socu_batched* h, int* counts, int* tiles) {
+  if (h == nullptr || counts == nullptr || tiles == nullptr) return fail(SOCU_EINVAL, "NULL argument");
+  for (int b = 0; b < kClasses; ++b) {
+    counts[b] = h->bucket_count[b];
+    tiles[b] = 8 << h->tile[b];
   }
   return SOCU_OK;
 }

@@ -1,19 +1,33 @@
 // Batched ND factor / solve for many independent systems of uneven size
-// (horizon N_p and block size n_p <= 32 per system), one persistent
-// workgroup per system.  The reference can only batch systems that share N
-// and n (block_tridiag_solver.py:749-768); this path has no counterpart there
-// but computes the same factor, in the same storage, per system.
+// (horizon N_p, block size n_p and right-hand sides m_p <= 32 per system),
+// one workgroup per system.  The reference can only batch systems that share
+// N and n (block_tridiag_solver.py:749-768); this path has no counterpart
+// there but computes, per system, the reference's factor and solve in the
+// reference's storage.
 //
-// A workgroup walks all ND levels of its system.  Within a level, the waves
-// share the node tasks (potrf + inverse, both trsm, y_i <- Linv y_i), a
-// barrier, then the receiving-block tasks (L_j -= R R^T + Lc^T Lc, fill-in
-// F -= R Lc, y_j -= R y_{j-s} + Lc^T y_{j+s}), and a barrier.  The backward
-// sweep walks the levels down with one task per node.  No two tasks of a
-// phase write the same block, so there are no atomics and no inter-workgroup
-// synchronisation.  Tiles are padded to NT in {8, 16, 32}; a wavefront runs
-// 64/(2*NT) node factorisations at once (tile_chol_inv_grouped).  Systems are
-// dispatched in order of decreasing cost (N n^3) so that long systems start
-// first.
+// Schedule.  The workgroup walks the levels of its system.  At level l
+// (s = 2^l, h = s/2) one phase runs, in rounds of TPW concurrent tasks:
+//   * node tasks, one per level-l node k.  For l > 0 the task first applies
+//     what the level l-1 elimination sends to node k: the Schur update
+//     L_k -= R_{k-h} R_{k-h}^T + Lc_{k+h}^T Lc_{k+h} and the two fill-ins
+//     that become node k's couplings, Lc_k -= R_{k-h} Lc_{k-h} and
+//     R_k -= R_{k+h} Lc_{k+h} (the syrk / gemm updates of level l-1 in
+//     block_tridiag_solver.py:797-813).  Then it factors and inverts L_k and
+//     forms R_k <- R_k L_k^-T, Lc_k <- L_k^-1 Lc_k and, for the forward
+//     sweep, y_k <- L_k^-1 (y_k - R_{k-h} y_{k-h} - Lc_{k+h}^T y_{k+h});
+//   * deferred target tasks, one per level l-1 Schur target j that is not a
+//     level-l node (j = 2s(v+1)-1): L_j -= R_{j-h} R_{j-h}^T +
+//     Lc_{j+h}^T Lc_{j+h} and the matching y_j update.
+// No two tasks of a level write the same block and every task reads only
+// blocks finished at an earlier level, so a level is one barrier, with no
+// atomics.  The backward sweep walks the levels down, one task per node:
+// x_i <- L_i^-T (x_i - R_i^T x_{i+s} - Lc_i x_{i-s}).
+//
+// Tiles are NT x (NT+1) in LDS, NT in {8, 16, 32}, zero padded beyond n (the
+// diagonal block identity padded).  A workgroup is 256 threads and runs
+// TPW = 1024 / NT^2 tasks at once (1, 4, 16); block products run on MFMA
+// (NT = 32: each wavefront owns one 16x16 output tile; NT = 16: one
+// wavefront per task) or, for NT = 8, as 2x2 register blocks per lane.
 #include "../../include/socu_hip.h"
 #include "nd_common.h"
 #include "nd_internal.h"
@@ -21,315 +35,402 @@
 
 namespace socu {
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+namespace {
+
+constexpr int kBT = 256;  // threads per workgroup
 
 template <typename T, int NT>
-struct GroupScratch {
-  T A[NT * (NT + 1)];
+struct BSlot {
+  T A[NT * (NT + 1)];  // diagonal block being factored
   T B[NT * (NT + 1)];
-  T col[NT];
+  T C[NT * (NT + 1)];
+  T D[NT * (NT + 1)];
+  T col[NT];  // column broadcast buffer
   T dinv[NT];
 };
 
-// Load an n x n row-major block into an NT x (NT+1) tile; entries outside n
-// become the identity (diag) or zero.
+template <int NT>
+struct BShape {
+  static constexpr int TP = NT + 1;
+  static constexpr int LPT = NT * NT / 4;  // lanes per task
+  static constexpr int TPW = kBT / LPT;    // tasks per workgroup
+};
+
+// Block `src` (n x n, row-major) into tile t; outside n: identity (ident) or 0.
 template <typename T, int NT>
-__device__ __forceinline__ void grp_load(T* tile, const T* __restrict__ src, int n, int gl, bool ident) {
-  constexpr int TP = NT + 1;
-  for (int idx = gl; idx < NT * NT; idx += 2 * NT) {
-    const int r = idx / NT, c = idx - r * NT;
-    tile[r * TP + c] = (r < n && c < n) ? src[(size_t)r * n + c] : ((ident && r == c) ? T(1) : T(0));
+__device__ __forceinline__ void tload(T* t, const T* __restrict__ src, int n, int q, bool ident) {
+  constexpr int TP = BShape<NT>::TP, LPT = BShape<NT>::LPT;
+  T v[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int idx = q + e * LPT, r = idx / NT, c = idx % NT;
+    v[e] = (src != nullptr && r < n && c < n) ? src[(size_t)r * n + c] : ((ident && r == c) ? T(1) : T(0));
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int idx = q + e * LPT, r = idx / NT, c = idx % NT;
+    t[r * TP + c] = v[e];
   }
 }
 
+// Vector block (n x m, row-major) into a tile of pitch yp; rows >= n zero.
 template <typename T, int NT>
-__device__ __forceinline__ void grp_load_rhs(T* tile, const T* __restrict__ src, int n, int m, int gl) {
-  constexpr int TP = NT + 1;
-  for (int idx = gl; idx < NT * m; idx += 2 * NT) {
+__device__ __forceinline__ void yload(T* t, int yp, const T* __restrict__ src, int n, int m, int q) {
+  constexpr int LPT = BShape<NT>::LPT;
+  for (int idx = q; idx < NT * m; idx += LPT) {
     const int r = idx / m, c = idx - r * m;
-    tile[r * TP + c] = (r < n) ? src[(size_t)r * m + c] : T(0);
+    t[r * yp + c] = (src != nullptr && r < n) ? src[(size_t)r * m + c] : T(0);
   }
 }
 
-template <typename T, int NT, int W>
-__device__ __forceinline__ void batched_system(const BatchProblem P, int op, GroupScratch<T, NT>* scr) {
-  constexpr int G = 64 / (2 * NT);
-  constexpr int TP = NT + 1;
-  const int wave = wave_id();
-  const int lane = lane_id();
-  const int g = lane / (2 * NT);
-  const int gl = lane % (2 * NT);
-  const bool rowl = gl < NT;
-  const int r = gl % NT;
-  GroupScratch<T, NT>& my = scr[wave * G + g];
+// C = A B (NT x NT, inner extent kend) with element functors fa(i, k) and
+// fb(k, j); a second product A2 B2 (extent kend2, when use2) is added.
+// out(i, j, v) receives each output element this thread owns (4 of them).
+// For NT >= 16 the predicates guarding a call must be uniform per wavefront.
+template <typename T, int NT, typename FA, typename FB, typename FA2, typename FB2, typename FO>
+__device__ __forceinline__ void tprod(int kend, FA fa, FB fb, bool use2, int kend2, FA2 fa2, FB2 fb2, FO out) {
+  if constexpr (NT == 8) {
+    const int q = threadIdx.x & 15;
+    const int i = 2 * (q >> 2), j = 2 * (q & 3);
+    T a00 = T(0), a01 = T(0), a10 = T(0), a11 = T(0);
+    for (int k = 0; k < kend; ++k) {
+      const T x0 = fa(i, k), x1 = fa(i + 1, k), y0 = fb(k, j), y1 = fb(k, j + 1);
+      a00 += x0 * y0;
+      a01 += x0 * y1;
+      a10 += x1 * y0;
+      a11 += x1 * y1;
+    }
+    if (use2)
+      for (int k = 0; k < kend2; ++k) {
+        const T x0 = fa2(i, k), x1 = fa2(i + 1, k), y0 = fb2(k, j), y1 = fb2(k, j + 1);
+        a00 += x0 * y0;
+        a01 += x0 * y1;
+        a10 += x1 * y0;
+        a11 += x1 * y1;
+      }
+    out(i, j, a00);
+    out(i, j + 1, a01);
+    out(i + 1, j, a10);
+    out(i + 1, j + 1, a11);
+  } else {
+    const int l = lane_id();
+    int i0 = 0, j0 = 0;
+    if constexpr (NT == 32) {
+      i0 = 16 * (wave_id() >> 1);
+      j0 = 16 * (wave_id() & 1);
+    }
+    const int ii = i0 + (l & 15), jj = j0 + (l & 15), kq = l >> 4;
+    typename Mfma<T>::acc_t acc = acc_zero<T>();
+    for (int k = 0; k < kend; k += 4) acc = Mfma<T>::mma(fa(ii, k + kq), fb(k + kq, jj), acc);
+    if (use2)
+      for (int k = 0; k < kend2; k += 4) acc = Mfma<T>::mma(fa2(ii, k + kq), fb2(k + kq, jj), acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out(i0 + Mfma<T>::row(l, r), j0 + Mfma<T>::col(l, r), acc[r]);
+  }
+}
+
+template <typename T, int NT, typename FA, typename FB, typename FO>
+__device__ __forceinline__ void tprod1(int kend, FA fa, FB fb, FO out) {
+  tprod<T, NT>(kend, fa, fb, false, 0, fa, fb, out);
+}
+
+// Vector product Y (n x m) = A (n x kend) X (kend x m) plus an optional
+// second term; the task's lanes own 2x2 blocks of the NT x NT index space.
+template <typename T, int NT, typename FA, typename FX, typename FA2, typename FX2, typename FO>
+__device__ __forceinline__ void vprod(int n, int m, int kend, FA fa, FX fx, bool use2, int kend2, FA2 fa2, FX2 fx2,
+                                      FO out) {
+  constexpr int LPT = BShape<NT>::LPT;
+  const int q = threadIdx.x % LPT;
+  const int i = 2 * (q / (NT / 2)), j = 2 * (q % (NT / 2));
+  if (i >= n || j >= m) return;
+  const bool i1 = i + 1 < n, j1 = j + 1 < m;
+  T a00 = T(0), a01 = T(0), a10 = T(0), a11 = T(0);
+  for (int k = 0; k < kend; ++k) {
+    const T x0 = fa(i, k), x1 = i1 ? fa(i + 1, k) : T(0);
+    const T y0 = fx(k, j), y1 = j1 ? fx(k, j + 1) : T(0);
+    a00 += x0 * y0;
+    a01 += x0 * y1;
+    a10 += x1 * y0;
+    a11 += x1 * y1;
+  }
+  if (use2)
+    for (int k = 0; k < kend2; ++k) {
+      const T x0 = fa2(i, k), x1 = i1 ? fa2(i + 1, k) : T(0);
+      const T y0 = fx2(k, j), y1 = j1 ? fx2(k, j + 1) : T(0);
+      a00 += x0 * y0;
+      a01 += x0 * y1;
+      a10 += x1 * y0;
+      a11 += x1 * y1;
+    }
+  out(i, j, a00);
+  if (j1) out(i, j + 1, a01);
+  if (i1) out(i + 1, j, a10);
+  if (i1 && j1) out(i + 1, j + 1, a11);
+}
+
+// Cholesky + inverse of every slot's A tile (lower: L, strict upper: L^-T,
+// dinv: 1/L[r][r]).
+template <typename T, int NT>
+__device__ __forceinline__ void slots_chol(BSlot<T, NT>* slots) {
+  constexpr int TP = BShape<NT>::TP;
+  if constexpr (NT == 32) {
+    if (wave_id() == 0) tile_chol_inv<T>(slots[0].A, TP, slots[0].dinv, slots[0].col);
+  } else if constexpr (NT == 16) {
+    BSlot<T, NT>& S = slots[wave_id()];
+    if (lane_id() < 32) tile_chol_inv_grouped<T, 16>(S.A, S.col, S.dinv, 0, TP);
+  } else {
+    constexpr int stride = (int)(sizeof(BSlot<T, NT>) / sizeof(T));
+    static_assert(sizeof(BSlot<T, NT>) % sizeof(T) == 0, "slot stride");
+    BSlot<T, NT>& S = slots[wave_id() * 4];
+    tile_chol_inv_grouped<T, 8>(S.A, S.col, S.dinv, stride, TP);
+  }
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ void batched_system(const BatchProblem& P, int op, unsigned char* smem, int yp) {
+  constexpr int TP = BShape<NT>::TP, LPT = BShape<NT>::LPT, TPW = BShape<NT>::TPW;
+  BSlot<T, NT>* slots = reinterpret_cast<BSlot<T, NT>*>(smem);
+  const int slot = threadIdx.x / LPT, q = threadIdx.x % LPT;
+  BSlot<T, NT>& S = slots[slot];
+  T* Y0 = reinterpret_cast<T*>(slots + TPW) + (size_t)slot * 3 * NT * yp;
+  T* Y1 = Y0 + NT * yp;
+  T* Y2 = Y1 + NT * yp;
   const int N = P.N, n = P.n, m = P.m;
   const size_t nn = (size_t)n * n, nm = (size_t)n * m;
   T* L = static_cast<T*>(P.L);
   T* E = static_cast<T*>(P.E);
   T* x = static_cast<T*>(P.x);
-  const bool do_factor = op == SOCU_OP_FACTOR || op == SOCU_OP_FACTOR_AND_SOLVE;
-  const bool do_fwd = op == SOCU_OP_FACTOR_AND_SOLVE || op == SOCU_OP_SOLVE || op == SOCU_OP_FORWARD;
-  const bool do_bwd = op == SOCU_OP_FACTOR_AND_SOLVE || op == SOCU_OP_SOLVE || op == SOCU_OP_BACKWARD;
+  const bool fac = op == SOCU_OP_FACTOR || op == SOCU_OP_FACTOR_AND_SOLVE;
+  const bool fwd = op == SOCU_OP_FACTOR_AND_SOLVE || op == SOCU_OP_SOLVE || op == SOCU_OP_FORWARD;
+  const bool bwd = op == SOCU_OP_FACTOR_AND_SOLVE || op == SOCU_OP_SOLVE || op == SOCU_OP_BACKWARD;
   const int Lv = recursive_iterations(N);
-  const int slots = W * G;  // concurrent tasks per workgroup
-  constexpr int kStride = (int)(sizeof(GroupScratch<T, NT>) / sizeof(T));
-  static_assert(sizeof(GroupScratch<T, NT>) % sizeof(T) == 0, "scratch stride");
-  GroupScratch<T, NT>& w0 = scr[wave * G];
+  const int kp = (NT == 8) ? n : ((n + 3) & ~3);  // inner extent of tile products
+  T* At = S.A;
+  T* Bt = S.B;
+  T* Ct = S.C;
+  T* Dt = S.D;
+  T* dv = S.dinv;
+  // Linv[i][k] of a factored tile: its strict upper triangle holds Linv^T
+  auto linv = [&](int i, int k) -> T { return (k < i) ? At[k * TP + i] : ((k == i) ? dv[i] : T(0)); };
 
-  if (do_factor || do_fwd) {
-    int off = 0;
+  if (fac || fwd) {
+    int offp = 0, off = 0;  // E segment offsets of levels l-1 and l
     for (int l = 0; l < Lv; ++l) {
-      const int s = 1 << l;
+      const int s = 1 << l, h = s >> 1;
+      if (l > 0) {
+        offp = off;
+        off = off + (N >> (l - 1)) - 1;
+      }
       const int dim = (N + s) / (2 * s);
-      const int next_off = off + (N >> l) - 1;
-      // ---- node tasks ----
-      for (int base = 0; base < dim; base += slots) {
-        const int t = base + wave * G + g;
-        const bool real = t < dim;
-        const int i = s - 1 + 2 * s * t;
-        T* Li = L + (size_t)i * nn;
-        if (do_factor) {
-          if (real) grp_load<T, NT>(my.A, Li, n, gl, true);
-          else
-            for (int idx = gl; idx < NT * NT; idx += 2 * NT) my.A[(idx / NT) * TP + idx % NT] = (idx / NT == idx % NT);
-          wave_sync();
-          tile_chol_inv_grouped<T, NT>(w0.A, w0.col, w0.dinv, kStride, TP);
-          wave_sync();
-          if (real)
-            for (int idx = gl; idx < n * n; idx += 2 * NT) Li[idx] = my.A[(idx / n) * TP + idx % n];
-        } else if (real) {
-          // factor already in L: bring L_i (lower + stored inverse) into the tile
-          grp_load<T, NT>(my.A, Li, n, gl, true);
-          if (rowl) my.dinv[r] = (r < n) ? T(1) / Li[(size_t)r * n + r] : T(1);
-          wave_sync();
+      const int nB = (l > 0) ? N / (2 * s) : 0;
+      const int ntask = dim + nB;
+      for (int base = 0; base < ntask; base += TPW) {
+        const int t = base + slot;
+        const bool node = t < dim;
+        const bool defer = !node && t < ntask;
+        const bool task = node || defer;
+        const int k = node ? s - 1 + 2 * s * t : 2 * s * (t - dim + 1) - 1;  // block row written
+        const bool up = l > 0 && task;       // receives from level l-1
+        const bool right = up && k + h < N;  // Lc_{k+h} exists
+        const bool has_lc = node && k >= s;  // Lc_k exists (and, for l > 0, Lc_{k-h})
+        const bool has_r = node && k + s < N;  // R_k exists (and, for l > 0, R_{k+h})
+        const int sh = l - 1;                  // log2(h)
+        T* Lk = task ? L + (size_t)k * nn : nullptr;
+        T* Rk = has_r ? E + (size_t)(off + k / s) * nn : nullptr;
+        T* Lck = has_lc ? E + (size_t)(off + (k - s) / s) * nn : nullptr;
+        const T* Rl = up ? E + (size_t)(offp + ((k - h) >> sh)) * nn : nullptr;                // (k, k-h)
+        const T* Lcr = right ? E + (size_t)(offp + (k >> sh)) * nn : nullptr;                  // (k+h, k)
+        const T* Lcl = (up && has_lc) ? E + (size_t)(offp + ((k - s) >> sh)) * nn : nullptr;   // (k-h, k-s)
+        const T* Rr = (up && has_r) ? E + (size_t)(offp + ((k + h) >> sh)) * nn : nullptr;     // (k+s, k+h)
+        T* yk = (fwd && task) ? x + (size_t)k * nm : nullptr;
+
+        // ---- P1: loads ----
+        tload<T, NT>(At, Lk, n, q, true);
+        if (up) {
+          tload<T, NT>(Bt, Rl, n, q, false);
+          tload<T, NT>(Ct, Lcr, n, q, false);
+          if (fac) tload<T, NT>(Dt, Lcl, n, q, false);
+        } else if (node && fac) {  // level 0: the original couplings of node k
+          tload<T, NT>(Bt, Rk, n, q, false);
+          tload<T, NT>(Dt, Lck, n, q, false);
         }
-        if (real) {
-          // Linv[c][k] = A[k][c] (k < c), dinv[c] (k == c)
-          if (do_factor && i + s < N && rowl && r < n) {  // R <- R Linv^T, lane r holds row r of R
-            T* R = E + (size_t)(off + i / s) * nn;
-            T row[NT];
-#pragma unroll
-            for (int k = 0; k < NT; ++k) row[k] = (k < n) ? R[(size_t)r * n + k] : T(0);
-#pragma unroll
-            for (int c = 0; c < NT; ++c) {
-              T acc = row[c] * my.dinv[c];
-#pragma unroll
-              for (int k = 0; k < c; ++k) acc += row[k] * my.A[k * TP + c];
-              if (c < n) R[(size_t)r * n + c] = acc;
-            }
-          }
-          if (do_fwd) {  // y_i <- Linv y_i
-            grp_load_rhs<T, NT>(my.B, x + (size_t)i * nm, n, m, gl);
-            wave_sync();
-            if (rowl && r < n)
-              for (int c = 0; c < m; ++c) {
-                T acc = my.dinv[r] * my.B[r * TP + c];
-                for (int k = 0; k < r; ++k) acc += my.A[k * TP + r] * my.B[k * TP + c];
-                x[(size_t)i * nm + (size_t)r * m + c] = acc;
-              }
-            wave_sync();
-          }
-          if (do_factor && i >= s) {  // Lc <- Linv Lc
-            T* Lc = E + (size_t)(off + (i - s) / s) * nn;
-            grp_load<T, NT>(my.B, Lc, n, gl, false);
-            wave_sync();
-            if (rowl && r < n)
-              for (int c = 0; c < n; ++c) {
-                T acc = my.dinv[r] * my.B[r * TP + c];
-                for (int k = 0; k < r; ++k) acc += my.A[k * TP + r] * my.B[k * TP + c];
-                Lc[(size_t)r * n + c] = acc;
-              }
-            wave_sync();
+        if (fwd) {
+          yload<T, NT>(Y0, yp, yk, n, m, q);
+          if (up) {
+            yload<T, NT>(Y1, yp, x + (size_t)(k - h) * nm, n, m, q);
+            yload<T, NT>(Y2, yp, right ? x + (size_t)(k + h) * nm : nullptr, n, m, q);
           }
         }
+        if (!fac && node && q < NT) dv[q] = (q < n) ? T(1) / Lk[(size_t)q * n + q] : T(1);
+        __syncthreads();
+
+        // ---- P2: updates from level l-1 ----
+        T keep[4];
+        int ki[4], kj[4];
+        int nk = 0;
+        if (up) {
+          if (fac)
+            tprod<T, NT>(
+                kp, [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Bt[j * TP + kk]; },
+                right, kp, [&](int i, int kk) { return Ct[kk * TP + i]; },
+                [&](int kk, int j) { return Ct[kk * TP + j]; }, [&](int i, int j, T v) { At[i * TP + j] -= v; });
+          if (fwd)
+            vprod<T, NT>(
+                n, m, n, [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Y1[kk * yp + j]; },
+                right, n, [&](int i, int kk) { return Ct[kk * TP + i]; },
+                [&](int kk, int j) { return Y2[kk * yp + j]; }, [&](int i, int j, T v) { Y0[i * yp + j] -= v; });
+          if (fac && has_lc)
+            tprod1<T, NT>(
+                kp, [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Dt[kk * TP + j]; },
+                [&](int i, int j, T v) {
+                  keep[nk] = (i < n && j < n) ? Lck[(size_t)i * n + j] - v : T(0);
+                  ki[nk] = i;
+                  kj[nk] = j;
+                  ++nk;
+                });
+        }
+        __syncthreads();
+
+        // ---- P3: finish the fill-in of Lc_k, stage R_{k+h}; deferred stores ----
+        if (up && fac && has_lc)
+          for (int e = 0; e < nk; ++e) Dt[ki[e] * TP + kj[e]] = keep[e];
+        if (up && fac) tload<T, NT>(Bt, Rr, n, q, false);
+        if (defer) {
+          if (fac)
+            for (int idx = q; idx < n * n; idx += LPT) Lk[idx] = At[(idx / n) * TP + idx % n];
+          if (fwd)
+            for (int idx = q; idx < n * m; idx += LPT) yk[idx] = Y0[(idx / m) * yp + idx % m];
+        }
+        __syncthreads();
+
+        // ---- P4: fill-in of R_k (workgroup-uniform condition: barriers) ----
+        if (l > 0 && fac) {
+          nk = 0;
+          if (has_r)
+            tprod1<T, NT>(
+                kp, [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Ct[kk * TP + j]; },
+                [&](int i, int j, T v) {
+                  keep[nk] = (i < n && j < n) ? Rk[(size_t)i * n + j] - v : T(0);
+                  ki[nk] = i;
+                  kj[nk] = j;
+                  ++nk;
+                });
+          __syncthreads();
+          for (int e = 0; e < nk; ++e) Bt[ki[e] * TP + kj[e]] = keep[e];
+          __syncthreads();
+        }
+
+        // ---- P5: factor and invert the diagonal blocks ----
+        if (fac) {
+          slots_chol<T, NT>(slots);
+          __syncthreads();
+        }
+
+        // ---- P6: node outputs ----
+        if (node) {
+          if (fac) {
+            for (int idx = q; idx < n * n; idx += LPT) Lk[idx] = At[(idx / n) * TP + idx % n];
+            if (has_lc)  // Lc_k <- Linv_k Lc_k
+              tprod1<T, NT>(
+                  kp, linv, [&](int kk, int j) { return Dt[kk * TP + j]; },
+                  [&](int i, int j, T v) {
+                    if (i < n && j < n) Lck[(size_t)i * n + j] = v;
+                  });
+            if (has_r)  // R_k <- R_k Linv_k^T
+              tprod1<T, NT>(
+                  kp, [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return linv(j, kk); },
+                  [&](int i, int j, T v) {
+                    if (i < n && j < n) Rk[(size_t)i * n + j] = v;
+                  });
+          }
+          if (fwd)  // y_k <- Linv_k y_k
+            vprod<T, NT>(
+                n, m, n, linv, [&](int kk, int j) { return Y0[kk * yp + j]; }, false, 0, linv,
+                [&](int kk, int j) { return Y0[kk * yp + j]; }, [&](int i, int j, T v) { yk[(size_t)i * m + j] = v; });
+        }
+        __syncthreads();
       }
-      __syncthreads();
-      // ---- receiving-block tasks: targets j = 2s(u+1)-1, then fill-ins ----
-      const int nT = N / (2 * s);
-      int nF = 0;
-      if (do_factor)
-        for (int tt = 1; tt < dim; ++tt)
-          if (s - 1 + 2 * s * tt + s < N) ++nF;
-      const int ntask = nT + nF;
-      for (int base = 0; base < ntask; base += slots) {
-        const int u = base + wave * G + g;
-        if (u < nT) {
-          const int j = 2 * s * (u + 1) - 1;
-          const T* R = E + (size_t)(off + (j - s) / s) * nn;  // block (j, j-s)
-          const bool right = j + s < N;
-          const T* Lc = right ? E + (size_t)(off + j / s) * nn : nullptr;  // block (j+s, j)
-          grp_load<T, NT>(my.A, R, n, gl, false);
-          if (right) grp_load<T, NT>(my.B, Lc, n, gl, false);
-          wave_sync();
-          if (rowl && r < n) {
-            if (do_factor) {
-              T* Lj = L + (size_t)j * nn;
-              for (int c = 0; c <= r; ++c) {
-                T acc = T(0);
-                for (int k = 0; k < n; ++k) acc += my.A[r * TP + k] * my.A[c * TP + k];
-                if (right)
-                  for (int k = 0; k < n; ++k) acc += my.B[k * TP + r] * my.B[k * TP + c];
-                Lj[(size_t)r * n + c] -= acc;
-              }
-            }
-            if (do_fwd) {
-              const T* yl = x + (size_t)(j - s) * nm;
-              const T* yr = right ? x + (size_t)(j + s) * nm : nullptr;
-              for (int c = 0; c < m; ++c) {
-                T acc = T(0);
-                for (int k = 0; k < n; ++k) acc += my.A[r * TP + k] * yl[(size_t)k * m + c];
-                if (right)
-                  for (int k = 0; k < n; ++k) acc += my.B[k * TP + r] * yr[(size_t)k * m + c];
-                x[(size_t)j * nm + (size_t)r * m + c] -= acc;
-              }
-            }
-          }
-          wave_sync();
-        } else if (u < ntask) {
-          const int tt = u - nT + 1;
-          const int i = s - 1 + 2 * s * tt;
-          const T* R = E + (size_t)(off + i / s) * nn;
-          const T* Lc = E + (size_t)(off + (i - s) / s) * nn;
-          T* F = E + (size_t)(next_off + (i - s) / (2 * s)) * nn;
-          grp_load<T, NT>(my.A, R, n, gl, false);
-          grp_load<T, NT>(my.B, Lc, n, gl, false);
-          wave_sync();
-          if (rowl && r < n)
-            for (int c = 0; c < n; ++c) {
-              T acc = T(0);
-              for (int k = 0; k < n; ++k) acc += my.A[r * TP + k] * my.B[k * TP + c];
-              F[(size_t)r * n + c] -= acc;
-            }
-          wave_sync();
-        }
-      }
-      __syncthreads();
-      off = next_off;
     }
   }
 
-  if (do_bwd) {
+  if (bwd) {
     for (int l = Lv - 1; l >= 0; --l) {
       const int s = 1 << l;
       const int dim = (N + s) / (2 * s);
       int off = 0;
-      for (int q = 0; q < l; ++q) off += (N >> q) - 1;
-      for (int base = 0; base < dim; base += slots) {
-        const int t = base + wave * G + g;
-        if (t < dim) {
-          const int i = s - 1 + 2 * s * t;
-          const T* Li = L + (size_t)i * nn;
-          const bool right = i + s < N, left = i >= s;
-          // A <- R (block (i+s, i)) used as R^T; B <- updated x_i
-          if (right) grp_load<T, NT>(my.A, E + (size_t)(off + i / s) * nn, n, gl, false);
-          grp_load_rhs<T, NT>(my.B, x + (size_t)i * nm, n, m, gl);
-          wave_sync();
-          // x_i -= R^T x_{i+s} + Lc x_{i-s}: each row lane updates its own row of B
-          if (rowl && r < n) {
-            const T* Lc = left ? E + (size_t)(off + (i - s) / s) * nn : nullptr;  // block (i, i-s)
-            const T* xr = right ? x + (size_t)(i + s) * nm : nullptr;
-            const T* xl = left ? x + (size_t)(i - s) * nm : nullptr;
-            for (int c = 0; c < m; ++c) {
-              T acc = my.B[r * TP + c];
-              if (right)
-                for (int k = 0; k < n; ++k) acc -= my.A[k * TP + r] * xr[(size_t)k * m + c];
-              if (left)
-                for (int k = 0; k < n; ++k) acc -= Lc[(size_t)r * n + k] * xl[(size_t)k * m + c];
-              my.B[r * TP + c] = acc;
-            }
-          }
-          wave_sync();
-          // x_i <- Linv^T x_i:  out[r] = x[r] / L[r][r] + sum_{k>r} U[r][k] x[k]
-          if (rowl && r < n) {
-            const T dg = Li[(size_t)r * n + r];
-            for (int c = 0; c < m; ++c) {
-              T acc = my.B[r * TP + c] / dg;
-              for (int k = r + 1; k < n; ++k) acc += Li[(size_t)r * n + k] * my.B[k * TP + c];
-              x[(size_t)i * nm + (size_t)r * m + c] = acc;
-            }
-          }
-          wave_sync();
-        }
+      for (int u = 0; u < l; ++u) off += (N >> u) - 1;
+      for (int base = 0; base < dim; base += TPW) {
+        const int t = base + slot;
+        const bool node = t < dim;
+        const int i = s - 1 + 2 * s * t;
+        const bool right = node && i + s < N, left = node && i >= s;
+        T* Li = node ? L + (size_t)i * nn : nullptr;
+        T* xi = node ? x + (size_t)i * nm : nullptr;
+        tload<T, NT>(At, Li, n, q, true);
+        tload<T, NT>(Bt, right ? E + (size_t)(off + i / s) * nn : nullptr, n, q, false);       // R_i
+        tload<T, NT>(Ct, left ? E + (size_t)(off + (i - s) / s) * nn : nullptr, n, q, false);  // Lc_i
+        yload<T, NT>(Y0, yp, xi, n, m, q);
+        yload<T, NT>(Y1, yp, right ? x + (size_t)(i + s) * nm : nullptr, n, m, q);
+        yload<T, NT>(Y2, yp, left ? x + (size_t)(i - s) * nm : nullptr, n, m, q);
+        if (node && q < NT) dv[q] = (q < n) ? T(1) / Li[(size_t)q * n + q] : T(1);
+        __syncthreads();
+        // x_i - R_i^T x_{i+s} - Lc_i x_{i-s}
+        T keep[4];
+        int ki[4], kj[4];
+        int nk = 0;
+        if (node)
+          vprod<T, NT>(
+              n, m, n, [&](int r, int kk) { return Bt[kk * TP + r]; }, [&](int kk, int j) { return Y1[kk * yp + j]; },
+              true, n, [&](int r, int kk) { return Ct[r * TP + kk]; }, [&](int kk, int j) { return Y2[kk * yp + j]; },
+              [&](int r, int j, T v) {
+                keep[nk] = Y0[r * yp + j] - v;
+                ki[nk] = r;
+                kj[nk] = j;
+                ++nk;
+              });
+        __syncthreads();
+        for (int e = 0; e < nk; ++e) Y0[ki[e] * yp + kj[e]] = keep[e];
+        __syncthreads();
+        // x_i <- Linv_i^T x_i
+        if (node)
+          vprod<T, NT>(
+              n, m, n, [&](int r, int kk) { return linv(kk, r); }, [&](int kk, int j) { return Y0[kk * yp + j]; },
+              false, 0, [&](int r, int kk) { return linv(kk, r); }, [&](int kk, int j) { return Y0[kk * yp + j]; },
+              [&](int r, int j, T v) { xi[(size_t)r * m + j] = v; });
+        __syncthreads();
       }
-      __syncthreads();
     }
   }
 }
 
-// One workgroup of W wavefronts per system of tile class NT; `order` lists
-// the systems of the class by decreasing cost.
-template <typename T, int NT, int W>
-__global__ __launch_bounds__(64 * W) void k_batched(const BatchProblem* __restrict__ probs,
-                                                   const int* __restrict__ order, const uint8_t* __restrict__ mask,
-                                                   int op) {
+// One workgroup per system of tile class NT; `order` lists the systems.
+template <typename T, int NT>
+__global__ __launch_bounds__(kBT) void k_batched(const BatchProblem* __restrict__ probs, const int* __restrict__ order,
+                                                 const uint8_t* __restrict__ mask, int op, int yp) {
   const int p = order[blockIdx.x];
   if (masked_out(mask, p)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  batched_system<T, NT, W>(probs[p], op, reinterpret_cast<GroupScratch<T, NT>*>(smem_raw));
-}
-
-template <typename T, int NT, int W>
-constexpr size_t batched_lds() {
-  return sizeof(GroupScratch<T, NT>) * W * (64 / (2 * NT));
-}
-
-template <typename T, int NT, int W>
-static hipError_t set_attr() {
-  return hipFuncSetAttribute((const void*)k_batched<T, NT, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)batched_lds<T, NT, W>());
+  const BatchProblem P = probs[p];
+  batched_system<T, NT>(P, op, smem_raw, yp);
 }
 
 template <typename T, int NT>
-static hipError_t set_attrs_nt() {
-  hipError_t e = set_attr<T, NT, 1>();
-  if (e == hipSuccess) e = set_attr<T, NT, 2>();
-  if (e == hipSuccess) e = set_attr<T, NT, 4>();
-  return e;
-}
-
-hipError_t init_batched_kernels() {
-  hipError_t e = set_attrs_nt<double, 8>();
-  if (e == hipSuccess) e = set_attrs_nt<double, 16>();
-  if (e == hipSuccess) e = set_attrs_nt<double, 32>();
-  if (e == hipSuccess) e = set_attrs_nt<float, 8>();
-  if (e == hipSuccess) e = set_attrs_nt<float, 16>();
-  if (e == hipSuccess) e = set_attrs_nt<float, 32>();
-  return e;
-}
-
-int batched_class(int n) { return n <= 8 ? 0 : (n <= 16 ? 1 : 2); }
-
-template <typename T, int NT, int W>
-static void launch_w(const BatchProblem* dprobs, const int* dorder, int count, const uint8_t* mask, int op,
-                     hipStream_t st) {
-  const size_t lds = batched_lds<T, NT, W>();
-  hipLaunchKernelGGL((k_batched<T, NT, W>), dim3(count), dim3(64 * W), lds, st, dprobs, dorder, mask, op);
+size_t lds_bytes(int yp) {
+  return BShape<NT>::TPW * (sizeof(BSlot<T, NT>) + (size_t)3 * NT * yp * sizeof(T));
 }
 
 template <typename T, int NT>
-static void launch_nt(const BatchProblem* dprobs, const int* dorder, int count, int waves, const uint8_t* mask,
-                      int op, hipStream_t st) {
-  if (waves == 1) launch_w<T, NT, 1>(dprobs, dorder, count, mask, op, st);
-  else if (waves == 2) launch_w<T, NT, 2>(dprobs, dorder, count, mask, op, st);
-  else launch_w<T, NT, 4>(dprobs, dorder, count, mask, op, st);
+hipError_t set_attr() {
+  return hipFuncSetAttribute((const void*)k_batched<T, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds_bytes<T, NT>(NT + 1));
 }
 
-hipError_t batched_launch_class(const BatchProblem* dprobs, const int* dorder, int cls, int count, int waves,
-                                const uint8_t* mask, int op, int dtype, hipStream_t st) {
-  if (count <= 0) return hipSuccess;
-  if (dtype == SOCU_DTYPE_F64) {
-    if (cls == 0) launch_nt<double, 8>(dprobs, dorder, count, waves, mask, op, st);
-    else if (cls == 1) launch_nt<double, 16>(dprobs, dorder, count, waves, mask, op, st);
-    else launch_nt<double, 32>(dprobs, dorder, count, waves, mask, op, st);
-  } else {
-    if (cls == 0) launch_nt<float, 8>(dprobs, dorder, count, waves, mask, op, st);
-    else if (cls == 1) launch_nt<float, 16>(dprobs, dorder, count, waves, mask, op, st);
-    else launch_nt<float, 32>(dprobs, dorder, count, waves, mask, op, st);
-  }
-  return hipGetLastError();
+template <typename T, int NT>
+void launch_nt(const BatchProblem* dprobs, const int* dorder, int count, int mmax, const uint8_t* mask, int op,
+               hipStream_t st) {
+  const int yp = mmax + 1;
+  const size_t lds = lds_bytes<T, NT>(yp);
+  hipLaunchKernelGGL((k_batched<T, NT>), dim3(count), dim3(kBT), lds, st, dprobs, dorder, mask, op, yp);
 }
 
 // Copies every system's L, E and x from src to dst (same shapes); one
@@ -344,6 +445,36 @@ __global__ __launch_bounds__(256) void k_copy_systems(const BatchProblem* __rest
   for (size_t k = threadIdx.x; k < nL; k += blockDim.x) static_cast<T*>(b.L)[k] = static_cast<const T*>(a.L)[k];
   for (size_t k = threadIdx.x; k < nE; k += blockDim.x) static_cast<T*>(b.E)[k] = static_cast<const T*>(a.E)[k];
   for (size_t k = threadIdx.x; k < nx; k += blockDim.x) static_cast<T*>(b.x)[k] = static_cast<const T*>(a.x)[k];
+}
+
+}  // namespace
+
+hipError_t init_batched_kernels() {
+  hipError_t e = set_attr<double, 8>();
+  if (e == hipSuccess) e = set_attr<double, 16>();
+  if (e == hipSuccess) e = set_attr<double, 32>();
+  if (e == hipSuccess) e = set_attr<float, 8>();
+  if (e == hipSuccess) e = set_attr<float, 16>();
+  if (e == hipSuccess) e = set_attr<float, 32>();
+  return e;
+}
+
+int batched_class(int n) { return n <= 8 ? 0 : (n <= 16 ? 1 : 2); }
+
+hipError_t batched_launch_class(const BatchProblem* dprobs, const int* dorder, int cls, int count, int mmax,
+                                const uint8_t* mask, int op, int dtype, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  if (cls < 0 || cls > 2 || mmax < 1 || mmax > (8 << cls)) return hipErrorInvalidValue;
+  if (dtype == SOCU_DTYPE_F64) {
+    if (cls == 0) launch_nt<double, 8>(dprobs, dorder, count, mmax, mask, op, st);
+    else if (cls == 1) launch_nt<double, 16>(dprobs, dorder, count, mmax, mask, op, st);
+    else launch_nt<double, 32>(dprobs, dorder, count, mmax, mask, op, st);
+  } else {
+    if (cls == 0) launch_nt<float, 8>(dprobs, dorder, count, mmax, mask, op, st);
+    else if (cls == 1) launch_nt<float, 16>(dprobs, dorder, count, mmax, mask, op, st);
+    else launch_nt<float, 32>(dprobs, dorder, count, mmax, mask, op, st);
+  }
+  return hipGetLastError();
 }
 
 hipError_t batched_copy(const BatchProblem* dsrc, const BatchProblem* ddst, int count, int op, int dtype,
