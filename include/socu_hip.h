@@ -132,15 +132,17 @@ int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask
  * n <= socu_batched_max_block_size() and m <= 32.  mask is NULL or `count`
  * bytes indexed like `systems`.  One workgroup per system walks all its
  * levels.  Systems fall in size buckets by max(n, m) (<= 8, <= 16, <= 32);
- * each bucket runs with a tile size of at least its bound, one launch per
- * tile size, the launches on forked streams.  With `tune` the create call
- * times every admissible tile size per bucket on a device copy of the batch
- * (the caller's arrays are only read) and keeps the fastest; results are
- * cached per process and, if the environment variable SOCU_TUNE_CACHE names
- * a file, appended there and reused by later processes.  Without `tune` each
- * bucket uses its own bound.  create copies the system table to the device
- * and synchronises; execute allocates nothing and synchronises nothing.
- * socu_batched_config returns per bucket the system count and tile size. */
+ * each bucket is one launch, with a tile size of at least its bound and
+ * workgroups of 1 or 4 wavefronts, the launches on forked streams.  With
+ * `tune` the create call times every admissible (tile size, wavefronts) per
+ * bucket on a device copy of the batch (the caller's arrays are only read)
+ * and keeps the fastest; results are cached per process and, if the
+ * environment variable SOCU_TUNE_CACHE names a file, appended there and
+ * reused by later processes.  Without `tune` each bucket uses its own bound
+ * and one wavefront.  create copies the system table to the device and
+ * synchronises; execute allocates nothing and synchronises nothing.
+ * socu_batched_config returns per bucket the system count, tile size and
+ * wavefronts per workgroup. */
 typedef struct socu_batch_system {
   void* L;
   void* E;
@@ -155,7 +157,7 @@ int socu_batched_max_block_size(void);
 int socu_batched_create(socu_batched** out, int op, const socu_batch_system* systems, int count,
                         const unsigned char* mask, int dtype, int tune, int use_graph);
 int socu_batched_execute(socu_batched* handle, void* stream);
-int socu_batched_config(socu_batched* handle, int* counts, int* tiles);
+int socu_batched_config(socu_batched* handle, int* counts, int* tiles, int* waves);
 int socu_batched_destroy(socu_batched* handle);
 
 /* Adjoint scatter (no counterpart in the reference, which has no gradient).

@@ -67,7 +67,7 @@ SIGNATURES = {
     "socu_batched_max_block_size": (_i, []),
     "socu_batched_create": (_i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(BatchSystem), _i, _vp, _i, _i, _i]),
     "socu_batched_execute": (_i, [_vp, _vp]),
-    "socu_batched_config": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "socu_batched_config": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "socu_batched_destroy": (_i, [_vp]),
     "socu_adjoint_scatter": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "socu_last_error": (ctypes.c_char_p, []),

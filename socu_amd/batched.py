@@ -73,12 +73,15 @@ class BatchedLaunch:
         _lib.check(rc, self._name)
 
     def config(self) -> dict:
-        """Per size bucket (max(n, m) <= 8, 16, 32): the number of systems and
-        the tile size they run with (as chosen by the autotuner when enabled)."""
+        """Per size bucket (max(n, m) <= 8, 16, 32): the number of systems, the
+        tile size and the wavefronts per workgroup they run with (as chosen
+        by the autotuner when enabled)."""
         counts = (ctypes.c_int * 3)()
         tiles = (ctypes.c_int * 3)()
-        _lib.check(self._lib.socu_batched_config(self._plan, counts, tiles), self._name)
-        return {bound: {"systems": counts[b], "tile": tiles[b]} for b, bound in enumerate((8, 16, 32))}
+        waves = (ctypes.c_int * 3)()
+        _lib.check(self._lib.socu_batched_config(self._plan, counts, tiles, waves), self._name)
+        return {bound: {"systems": counts[b], "tile": tiles[b], "waves": waves[b]}
+                for b, bound in enumerate((8, 16, 32))}
 
     def __del__(self):
         plan = getattr(self, "_plan", None)
@@ -101,9 +104,10 @@ def create_batched_cholesky_launch(L: Sequence[torch.Tensor], E: Sequence[torch.
     ``mask`` is a bool tensor with one entry per system; a False entry leaves
     that system untouched, as in the reference's mask (README.md:214-235).
     ``tune`` times every admissible tile size (8, 16, 32, at least the
-    bucket's bound) for each size bucket on a device copy of the batch when the
-    launch is created and keeps the fastest (cached per process, and in the
-    file named by SOCU_TUNE_CACHE).
+    bucket's bound) and workgroup size (1 or 4 wavefronts) for each size
+    bucket on a device copy of the batch when the launch is created and keeps
+    the fastest (cached per process, and in the file named by
+    SOCU_TUNE_CACHE).
     """
     if op not in _OPS:
         raise ValueError(f"unknown op {op!r}; expected one of {sorted(_OPS)}")

@@ -468,8 +468,9 @@ namespace {
 
 constexpr int kClasses = 3;  // tile classes NT = 8, 16, 32
 
-// Process-wide tuning results: key -> chosen tile class.  Optionally
-// persisted in the file named by SOCU_TUNE_CACHE ("key class" per line).
+// Process-wide tuning results: key -> chosen (tile class, wavefronts) coded
+// as 8 * waves + class.  Optionally persisted in the file named by
+// SOCU_TUNE_CACHE ("key code" per line).
 std::mutex g_tune_mu;
 std::map<std::string, int> g_tune;
 bool g_tune_loaded = false;
@@ -484,7 +485,7 @@ void tune_cache_load() {
   char key[256];
   int c = 0;
   while (fscanf(f, "%255s %d", key, &c) == 2)
-    if (c >= 0 && c < kClasses) g_tune[key] = c;
+    if (c >= 0 && (c & 7) < kClasses && ((c >> 3) == 1 || (c >> 3) == 4)) g_tune[key] = c;
   fclose(f);
 }
 
@@ -501,23 +502,24 @@ void tune_cache_store(const std::string& key, int c) {
 }  // namespace
 
 // Systems are grouped in size buckets b (max(n, m) <= 8, 16, 32); bucket b
-// runs with tile class tile[b] >= b.  Systems of buckets that share a tile
-// class form one launch.
+// runs as one launch with tile class tile[b] >= b and workgroups of
+// waves[b] wavefronts.
 struct socu_batched {
   int op = 0, dtype = 0, count = 0;
   const unsigned char* mask = nullptr;
   socu::BatchProblem* dprobs = nullptr;  // device copy of the system table
-  int* dorder = nullptr;                 // device: system indices grouped by launch class
+  int* dorder = nullptr;                 // device: system indices grouped by bucket
   std::vector<int> order;                // host copy of dorder
   std::vector<int> bucket_of;            // per system
   std::vector<long long> cost;           // per system
   std::vector<int> mrhs;                 // per system
-  int bucket_count[kClasses] = {0, 0, 0};
   int tile[kClasses] = {0, 1, 2};
-  int counts[kClasses] = {0, 0, 0};  // per launch class
+  int waves[kClasses] = {1, 1, 1};
+  int counts[kClasses] = {0, 0, 0};  // per bucket
   int starts[kClasses] = {0, 0, 0};
   int mmax[kClasses] = {1, 1, 1};
-  int launch_order[kClasses] = {2, 1, 0};  // launch classes by decreasing total cost
+  long long bcost[kClasses] = {0, 0, 0};
+  int launch_order[kClasses] = {2, 1, 0};  // buckets by decreasing total cost
   hipStream_t side[kClasses - 1] = {nullptr, nullptr};
   hipEvent_t fork = nullptr;
   hipEvent_t join[kClasses - 1] = {nullptr, nullptr};
@@ -541,35 +543,27 @@ void batched_release(socu_batched* h) {
   delete h;
 }
 
-// Host order of the systems of `buckets` (a bit set) by decreasing cost.
-std::vector<int> bucket_order(const socu_batched* h, unsigned buckets) {
-  std::vector<int> o;
-  for (int k = 0; k < h->count; ++k)
-    if (buckets & (1u << h->bucket_of[k])) o.push_back(k);
-  std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return h->cost[a] > h->cost[b]; });
-  return o;
-}
-
-// Builds the launch classes from h->tile and uploads the order.
+// Groups the systems by bucket (each by decreasing cost: long systems start
+// first) and uploads the order.
 int batched_layout(socu_batched* h) {
   h->order.clear();
-  long long ccost[kClasses] = {0, 0, 0};
-  for (int c = kClasses - 1; c >= 0; --c) {
-    unsigned set = 0;
-    for (int b = 0; b < kClasses; ++b)
-      if (h->tile[b] == c) set |= 1u << b;
-    const std::vector<int> o = bucket_order(h, set);
-    h->starts[c] = (int)h->order.size();
-    h->counts[c] = (int)o.size();
-    h->mmax[c] = 1;
+  for (int b = kClasses - 1; b >= 0; --b) {
+    std::vector<int> o;
+    for (int k = 0; k < h->count; ++k)
+      if (h->bucket_of[k] == b) o.push_back(k);
+    std::stable_sort(o.begin(), o.end(), [&](int x, int y) { return h->cost[x] > h->cost[y]; });
+    h->starts[b] = (int)h->order.size();
+    h->counts[b] = (int)o.size();
+    h->mmax[b] = 1;
+    h->bcost[b] = 0;
     for (int k : o) {
-      h->mmax[c] = std::max(h->mmax[c], h->mrhs[k]);
-      ccost[c] += h->cost[k];
+      h->mmax[b] = std::max(h->mmax[b], h->mrhs[k]);
+      h->bcost[b] += h->cost[k];
     }
     h->order.insert(h->order.end(), o.begin(), o.end());
   }
-  for (int c = 0; c < kClasses; ++c) h->launch_order[c] = c;
-  std::sort(h->launch_order, h->launch_order + kClasses, [&](int a, int b) { return ccost[a] > ccost[b]; });
+  for (int b = 0; b < kClasses; ++b) h->launch_order[b] = b;
+  std::sort(h->launch_order, h->launch_order + kClasses, [&](int x, int y) { return h->bcost[x] > h->bcost[y]; });
   const hipError_t e = hipMemcpy(h->dorder, h->order.data(), h->order.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "batched plan set-up (order)");
   return SOCU_OK;
@@ -591,39 +585,40 @@ int batched_enqueue(socu_batched* h, const unsigned char* mask, hipStream_t st) 
     if (nside == 0) e = hipEventRecord(h->fork, st);
     if (e == hipSuccess) e = hipStreamWaitEvent(h->side[nside], h->fork, 0);
     if (e == hipSuccess)
-      e = socu::batched_launch_class(h->dprobs, h->dorder + h->starts[c], c, h->counts[c], h->mmax[c], mask, h->op,
-                                     h->dtype, h->side[nside]);
+      e = socu::batched_launch_class(h->dprobs, h->dorder + h->starts[c], h->tile[c], h->counts[c], h->mmax[c],
+                                     h->waves[c], mask, h->op, h->dtype, h->side[nside]);
     if (e == hipSuccess) e = hipEventRecord(h->join[nside], h->side[nside]);
     ++nside;
   }
   if (e == hipSuccess && first >= 0)
-    e = socu::batched_launch_class(h->dprobs, h->dorder + h->starts[first], first, h->counts[first], h->mmax[first],
-                                   mask, h->op, h->dtype, st);
+    e = socu::batched_launch_class(h->dprobs, h->dorder + h->starts[first], h->tile[first], h->counts[first],
+                                   h->mmax[first], h->waves[first], mask, h->op, h->dtype, st);
   for (int k = 0; k < nside && e == hipSuccess; ++k) e = hipStreamWaitEvent(st, h->join[k], 0);
   if (e != hipSuccess) return hip_fail(e, "batched launch");
   return SOCU_OK;
 }
 
-// For each size bucket, times every admissible tile class (>= the bucket's
-// own) on a device copy of the batch and keeps the fastest.  The caller's
-// arrays are only read.
+// For each size bucket, times every admissible (tile class >= the bucket's
+// own, wavefronts 1 or 4) on a device copy of the batch and keeps the
+// fastest.  The caller's arrays are only read.  Needs batched_layout first.
 int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   tune_cache_load();
   std::string keys[kClasses];
   bool need = false;
   for (int b = 0; b < kClasses; ++b) {
-    if (h->bucket_count[b] == 0) continue;
-    long long bc = 0;
-    for (int k = 0; k < h->count; ++k)
-      if (h->bucket_of[k] == b) bc += h->cost[k];
+    if (h->counts[b] == 0) continue;
     char buf[160];
-    snprintf(buf, sizeof buf, "batched2:%s:op%d:bucket%d:count%d:cost%lld", h->dtype == SOCU_DTYPE_F64 ? "f64" : "f32",
-             h->op, b, h->bucket_count[b], bc);
+    snprintf(buf, sizeof buf, "batched3:%s:op%d:bucket%d:count%d:cost%lld", h->dtype == SOCU_DTYPE_F64 ? "f64" : "f32",
+             h->op, b, h->counts[b], h->bcost[b]);
     keys[b] = buf;
     auto it = g_tune.find(keys[b]);
-    if (it != g_tune.end() && it->second >= b) h->tile[b] = it->second;
-    else need = true;
+    if (it != g_tune.end() && (it->second & 7) >= b) {
+      h->tile[b] = it->second & 7;
+      h->waves[b] = it->second >> 3;
+    } else {
+      need = true;
+    }
   }
   if (!need) return SOCU_OK;
   const size_t es = h->dtype == SOCU_DTYPE_F64 ? sizeof(double) : sizeof(float);
@@ -639,7 +634,6 @@ int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) 
   }
   char* arena = nullptr;
   socu::BatchProblem* dcopy = nullptr;
-  int* dord = nullptr;
   hipStream_t ts = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = SOCU_OK;
@@ -656,45 +650,44 @@ int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) 
     }
     e = hipMalloc(&dcopy, copy.size() * sizeof(socu::BatchProblem));
   }
-  if (e == hipSuccess) e = hipMalloc(&dord, h->count * sizeof(int));
   if (e == hipSuccess) e = hipMemcpy(dcopy, copy.data(), copy.size() * sizeof(socu::BatchProblem), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ts, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   for (int b = 0; b < kClasses && e == hipSuccess; ++b) {
-    if (h->bucket_count[b] == 0 || (g_tune.count(keys[b]) && g_tune[keys[b]] >= b)) continue;
-    const std::vector<int> o = bucket_order(h, 1u << b);
-    int mm = 1;
-    for (int k : o) mm = std::max(mm, h->mrhs[k]);
-    e = hipMemcpy(dord, o.data(), o.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (h->counts[b] == 0 || (g_tune.count(keys[b]) && (g_tune[keys[b]] & 7) >= b)) continue;
     float best = 1e30f;
-    int best_c = b;
+    int best_c = b, best_w = 1;
     for (int c = b; c < kClasses && e == hipSuccess; ++c) {
-      for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
-        e = socu::batched_copy(h->dprobs, dcopy, h->count, h->op, h->dtype, ts);
-        if (e == hipSuccess) e = hipEventRecord(e0, ts);
-        if (e == hipSuccess)
-          e = socu::batched_launch_class(dcopy, dord, c, (int)o.size(), mm, nullptr, h->op, h->dtype, ts);
-        if (e == hipSuccess) e = hipEventRecord(e1, ts);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        if (e == hipSuccess && ms < best) {
-          best = ms;
-          best_c = c;
+      for (int w : {1, 4}) {
+        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+          e = socu::batched_copy(h->dprobs, dcopy, h->count, h->op, h->dtype, ts);
+          if (e == hipSuccess) e = hipEventRecord(e0, ts);
+          if (e == hipSuccess)
+            e = socu::batched_launch_class(dcopy, h->dorder + h->starts[b], c, h->counts[b], h->mmax[b], w, nullptr,
+                                           h->op, h->dtype, ts);
+          if (e == hipSuccess) e = hipEventRecord(e1, ts);
+          if (e == hipSuccess) e = hipEventSynchronize(e1);
+          float ms = 0.f;
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+          if (e == hipSuccess && ms < best) {
+            best = ms;
+            best_c = c;
+            best_w = w;
+          }
         }
       }
     }
     if (e == hipSuccess) {
       h->tile[b] = best_c;
-      tune_cache_store(keys[b], best_c);
+      h->waves[b] = best_w;
+      tune_cache_store(keys[b], 8 * best_w + best_c);
     }
   }
   if (e != hipSuccess) rc = hip_fail(e, "batched autotune");
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (ts) (void)hipStreamDestroy(ts);
-  if (dord) (void)hipFree(dord);
   if (dcopy) (void)hipFree(dcopy);
   if (arena) (void)hipFree(arena);
   return rc;
@@ -743,7 +736,6 @@ int socu_batched_create(socu_batched** out, int op, const socu_batch_system* sys
     h->cost[k] = (long long)S.N * S.n * S.n * (S.n + m);
     h->bucket_of[k] = socu::batched_class(std::max(S.n, m));
     h->mrhs[k] = m;
-    h->bucket_count[h->bucket_of[k]] += 1;
   }
   if (count == 0) {
     *out = h;
@@ -766,8 +758,8 @@ int socu_batched_create(socu_batched** out, int op, const socu_batch_system* sys
     batched_release(h);
     return hip_fail(e, "batched plan set-up");
   }
-  if (tune) rc = batched_tune(h, probs);
-  if (rc == SOCU_OK) rc = batched_layout(h);
+  rc = batched_layout(h);
+  if (rc == SOCU_OK && tune) rc = batched_tune(h, probs);
   if (rc) {
     batched_release(h);
     return rc;
@@ -817,11 +809,13 @@ int socu_batched_execute(socu_batched* h, void* stream) {
   return batched_enqueue(h, h->mask, st);
 }
 
-int socu_batched_config(socu_batched* h, int* counts, int* tiles) {
-  if (h == nullptr || counts == nullptr || tiles == nullptr) return fail(SOCU_EINVAL, "NULL argument");
+int socu_batched_config(socu_batched* h, int* counts, int* tiles, int* waves) {
+  if (h == nullptr || counts == nullptr || tiles == nullptr || waves == nullptr)
+    return fail(SOCU_EINVAL, "NULL argument");
   for (int b = 0; b < kClasses; ++b) {
-    counts[b] = h->bucket_count[b];
+    counts[b] = h->counts[b];
     tiles[b] = 8 << h->tile[b];
+    waves[b] = h->waves[b];
   }
   return SOCU_OK;
 }

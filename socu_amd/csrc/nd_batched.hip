@@ -24,10 +24,11 @@
 // x_i <- L_i^-T (x_i - R_i^T x_{i+s} - Lc_i x_{i-s}).
 //
 // Tiles are NT x (NT+1) in LDS, NT in {8, 16, 32}, zero padded beyond n (the
-// diagonal block identity padded).  A workgroup is 256 threads and runs
-// TPW = 1024 / NT^2 tasks at once (1, 4, 16); block products run on MFMA
-// (NT = 32: each wavefront owns one 16x16 output tile; NT = 16: one
-// wavefront per task) or, for NT = 8, as 2x2 register blocks per lane.
+// diagonal block identity padded).  A workgroup has W = 1 or 4 wavefronts and
+// runs one or more tasks at once (BShape); block products run on MFMA
+// (16x16 tiles, NT >= 16) or, for NT = 8, as 2x2 register blocks per lane.
+// Small workgroups (W = 1) let several systems share a compute unit, one per
+// SIMD; the tuner picks W per size bucket.
 #include "../../include/socu_hip.h"
 #include "nd_common.h"
 #include "nd_internal.h"
@@ -36,8 +37,6 @@
 namespace socu {
 
 namespace {
-
-constexpr int kBT = 256;  // threads per workgroup
 
 template <typename T, int NT>
 struct BSlot {
@@ -49,136 +48,158 @@ struct BSlot {
   T dinv[NT];
 };
 
-template <int NT>
+// Workgroups of W wavefronts.  A task uses LPT = min(64 W, NT^2/4) lanes
+// (at most four tile elements per lane) and the workgroup runs TPW tasks at
+// once: NT = 32: one task (W = 1: one wavefront does everything); NT = 16:
+// one task per wavefront; NT = 8: four tasks per wavefront.
+template <int NT, int W>
 struct BShape {
   static constexpr int TP = NT + 1;
-  static constexpr int LPT = NT * NT / 4;  // lanes per task
-  static constexpr int TPW = kBT / LPT;    // tasks per workgroup
+  static constexpr int BT = 64 * W;                  // threads per workgroup
+  static constexpr int NAT = NT * NT / 4;            // lanes at 4 outputs per lane
+  static constexpr int LPT = NAT < BT ? NAT : BT;    // lanes per task
+  static constexpr int TPW = BT / LPT;               // tasks per workgroup
+  static constexpr int EPL = NT * NT / LPT;          // tile elements per lane
+  static constexpr int KC = LPT / NT;                // lanes per row in vector products
+  static_assert(BT % LPT == 0 && (NT * NT) % LPT == 0 && LPT % NT == 0, "shape");
 };
 
 // Block `src` (n x n, row-major) into tile t; outside n: identity (ident) or 0.
-template <typename T, int NT>
+template <typename T, int NT, int W>
 __device__ __forceinline__ void tload(T* t, const T* __restrict__ src, int n, int q, bool ident) {
-  constexpr int TP = BShape<NT>::TP, LPT = BShape<NT>::LPT;
-  T v[4];
+  constexpr int TP = BShape<NT, W>::TP, LPT = BShape<NT, W>::LPT, EPL = BShape<NT, W>::EPL;
+  T v[EPL];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
+  for (int e = 0; e < EPL; ++e) {
     const int idx = q + e * LPT, r = idx / NT, c = idx % NT;
     v[e] = (src != nullptr && r < n && c < n) ? src[(size_t)r * n + c] : ((ident && r == c) ? T(1) : T(0));
   }
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
+  for (int e = 0; e < EPL; ++e) {
     const int idx = q + e * LPT, r = idx / NT, c = idx % NT;
     t[r * TP + c] = v[e];
   }
 }
 
 // Vector block (n x m, row-major) into a tile of pitch yp; rows >= n zero.
-template <typename T, int NT>
+template <typename T, int NT, int W>
 __device__ __forceinline__ void yload(T* t, int yp, const T* __restrict__ src, int n, int m, int q) {
-  constexpr int LPT = BShape<NT>::LPT;
+  constexpr int LPT = BShape<NT, W>::LPT;
   for (int idx = q; idx < NT * m; idx += LPT) {
     const int r = idx / m, c = idx - r * m;
     t[r * yp + c] = (src != nullptr && r < n) ? src[(size_t)r * m + c] : T(0);
   }
 }
 
-// C = A B (NT x NT, inner extent kend) with element functors fa(i, k) and
-// fb(k, j); a second product A2 B2 (extent kend2, when use2) is added.
-// out(i, j, v) receives each output element this thread owns (4 of them).
-// For NT >= 16 the predicates guarding a call must be uniform per wavefront.
-template <typename T, int NT, typename FA, typename FB, typename FA2, typename FB2, typename FO>
-__device__ __forceinline__ void tprod(int kend, FA fa, FB fb, bool use2, int kend2, FA2 fa2, FB2 fb2, FO out) {
+// C = A B (NT x NT) with element functors fa(i, k), fb(k, j), plus a second
+// product A2 B2 when use2; out(i, j, v) receives every output element this
+// thread owns.  Tiles are zero padded to NT, so the inner extent is NT.
+// NT = 8: 2x2 register blocks per lane.  NT >= 16: 16x16 MFMA tiles, the
+// task's wavefronts taking tiles round robin, inner loop unrolled with two
+// accumulators.  For NT >= 16 the predicates guarding a call must be uniform
+// per wavefront.
+template <typename T, int NT, int W, typename FA, typename FB, typename FA2, typename FB2, typename FO>
+__device__ __forceinline__ void tprod(FA fa, FB fb, bool use2, FA2 fa2, FB2 fb2, FO out) {
   if constexpr (NT == 8) {
     const int q = threadIdx.x & 15;
     const int i = 2 * (q >> 2), j = 2 * (q & 3);
     T a00 = T(0), a01 = T(0), a10 = T(0), a11 = T(0);
-    for (int k = 0; k < kend; ++k) {
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
       const T x0 = fa(i, k), x1 = fa(i + 1, k), y0 = fb(k, j), y1 = fb(k, j + 1);
       a00 += x0 * y0;
       a01 += x0 * y1;
       a10 += x1 * y0;
       a11 += x1 * y1;
     }
-    if (use2)
-      for (int k = 0; k < kend2; ++k) {
+    if (use2) {
+#pragma unroll
+      for (int k = 0; k < NT; ++k) {
         const T x0 = fa2(i, k), x1 = fa2(i + 1, k), y0 = fb2(k, j), y1 = fb2(k, j + 1);
         a00 += x0 * y0;
         a01 += x0 * y1;
         a10 += x1 * y0;
         a11 += x1 * y1;
       }
+    }
     out(i, j, a00);
     out(i, j + 1, a01);
     out(i + 1, j, a10);
     out(i + 1, j + 1, a11);
   } else {
+    constexpr int LPT = BShape<NT, W>::LPT;
+    constexpr int TD = NT / 16, TT = TD * TD;  // 16x16 tiles per side / per task
+    constexpr int WPT = LPT / 64;              // wavefronts per task
     const int l = lane_id();
-    int i0 = 0, j0 = 0;
-    if constexpr (NT == 32) {
-      i0 = 16 * (wave_id() >> 1);
-      j0 = 16 * (wave_id() & 1);
-    }
-    const int ii = i0 + (l & 15), jj = j0 + (l & 15), kq = l >> 4;
-    typename Mfma<T>::acc_t acc = acc_zero<T>();
-    for (int k = 0; k < kend; k += 4) acc = Mfma<T>::mma(fa(ii, k + kq), fb(k + kq, jj), acc);
-    if (use2)
-      for (int k = 0; k < kend2; k += 4) acc = Mfma<T>::mma(fa2(ii, k + kq), fb2(k + kq, jj), acc);
+    const int wt = (int)(threadIdx.x % LPT) / 64;
+    const int kq = l >> 4;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) out(i0 + Mfma<T>::row(l, r), j0 + Mfma<T>::col(l, r), acc[r]);
+    for (int tt = 0; tt < TT / WPT; ++tt) {
+      const int t = wt + tt * WPT;
+      const int i0 = 16 * (t / TD), j0 = 16 * (t % TD);
+      const int ii = i0 + (l & 15), jj = j0 + (l & 15);
+      typename Mfma<T>::acc_t a0 = acc_zero<T>(), a1 = acc_zero<T>();
+#pragma unroll
+      for (int k = 0; k < NT; k += 8) {
+        a0 = Mfma<T>::mma(fa(ii, k + kq), fb(k + kq, jj), a0);
+        a1 = Mfma<T>::mma(fa(ii, k + 4 + kq), fb(k + 4 + kq, jj), a1);
+      }
+      if (use2) {
+#pragma unroll
+        for (int k = 0; k < NT; k += 8) {
+          a0 = Mfma<T>::mma(fa2(ii, k + kq), fb2(k + kq, jj), a0);
+          a1 = Mfma<T>::mma(fa2(ii, k + 4 + kq), fb2(k + 4 + kq, jj), a1);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out(i0 + Mfma<T>::row(l, r), j0 + Mfma<T>::col(l, r), a0[r] + a1[r]);
+    }
   }
 }
 
-template <typename T, int NT, typename FA, typename FB, typename FO>
-__device__ __forceinline__ void tprod1(int kend, FA fa, FB fb, FO out) {
-  tprod<T, NT>(kend, fa, fb, false, 0, fa, fb, out);
+template <typename T, int NT, int W, typename FA, typename FB, typename FO>
+__device__ __forceinline__ void tprod1(FA fa, FB fb, FO out) {
+  tprod<T, NT, W>(fa, fb, false, fa, fb, out);
 }
 
 // Vector product Y (n x m) = A (n x kend) X (kend x m) plus an optional
-// second term; the task's lanes own 2x2 blocks of the NT x NT index space.
-template <typename T, int NT, typename FA, typename FX, typename FA2, typename FX2, typename FO>
+// second term.  KC consecutive lanes share row i (i < NT), each summing a
+// chunk of NT/KC inner indices, and a butterfly over those lanes completes
+// the sum; lane (i, chunk 0) calls out(i, j, v) for i < n, j < m.  Every
+// lane of the task must call it.
+template <typename T, int NT, int W, typename FA, typename FX, typename FA2, typename FX2, typename FO>
 __device__ __forceinline__ void vprod(int n, int m, int kend, FA fa, FX fx, bool use2, int kend2, FA2 fa2, FX2 fx2,
                                       FO out) {
-  constexpr int LPT = BShape<NT>::LPT;
+  constexpr int LPT = BShape<NT, W>::LPT, KC = BShape<NT, W>::KC, CS = NT / KC;
   const int q = threadIdx.x % LPT;
-  const int i = 2 * (q / (NT / 2)), j = 2 * (q % (NT / 2));
-  if (i >= n || j >= m) return;
-  const bool i1 = i + 1 < n, j1 = j + 1 < m;
-  T a00 = T(0), a01 = T(0), a10 = T(0), a11 = T(0);
-  for (int k = 0; k < kend; ++k) {
-    const T x0 = fa(i, k), x1 = i1 ? fa(i + 1, k) : T(0);
-    const T y0 = fx(k, j), y1 = j1 ? fx(k, j + 1) : T(0);
-    a00 += x0 * y0;
-    a01 += x0 * y1;
-    a10 += x1 * y0;
-    a11 += x1 * y1;
-  }
-  if (use2)
-    for (int k = 0; k < kend2; ++k) {
-      const T x0 = fa2(i, k), x1 = i1 ? fa2(i + 1, k) : T(0);
-      const T y0 = fx2(k, j), y1 = j1 ? fx2(k, j + 1) : T(0);
-      a00 += x0 * y0;
-      a01 += x0 * y1;
-      a10 += x1 * y0;
-      a11 += x1 * y1;
+  const int i = q / KC, k0 = CS * (q % KC);
+  for (int j = 0; j < m; ++j) {
+    T acc = T(0);
+#pragma unroll
+    for (int e = 0; e < CS; ++e)
+      if (k0 + e < kend) acc += fa(i, k0 + e) * fx(k0 + e, j);
+    if (use2) {
+#pragma unroll
+      for (int e = 0; e < CS; ++e)
+        if (k0 + e < kend2) acc += fa2(i, k0 + e) * fx2(k0 + e, j);
     }
-  out(i, j, a00);
-  if (j1) out(i, j + 1, a01);
-  if (i1) out(i + 1, j, a10);
-  if (i1 && j1) out(i + 1, j + 1, a11);
+#pragma unroll
+    for (int o = KC / 2; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (k0 == 0 && i < n) out(i, j, acc);
+  }
 }
 
 // Cholesky + inverse of every slot's A tile (lower: L, strict upper: L^-T,
 // dinv: 1/L[r][r]).
-template <typename T, int NT>
+template <typename T, int NT, int W>
 __device__ __forceinline__ void slots_chol(BSlot<T, NT>* slots) {
-  constexpr int TP = BShape<NT>::TP;
-  if constexpr (NT == 32) {
-    if (wave_id() == 0) tile_chol_inv<T>(slots[0].A, TP, slots[0].dinv, slots[0].col);
-  } else if constexpr (NT == 16) {
+  constexpr int TP = BShape<NT, W>::TP;
+  if constexpr (NT == 32) {  // one task per workgroup; its first wavefront
+    if (wave_id() == 0) tile_chol_inv32b<T>(slots[0].A, TP, slots[0].dinv, slots[0].col);
+  } else if constexpr (NT == 16) {  // one task per wavefront
     BSlot<T, NT>& S = slots[wave_id()];
-    if (lane_id() < 32) tile_chol_inv_grouped<T, 16>(S.A, S.col, S.dinv, 0, TP);
-  } else {
+    if (lane_id() < 32) tile_chol_inv16<T>(S.A, TP, S.dinv, S.col);
+  } else {  // four tasks per wavefront
     constexpr int stride = (int)(sizeof(BSlot<T, NT>) / sizeof(T));
     static_assert(sizeof(BSlot<T, NT>) % sizeof(T) == 0, "slot stride");
     BSlot<T, NT>& S = slots[wave_id() * 4];
@@ -186,9 +207,9 @@ __device__ __forceinline__ void slots_chol(BSlot<T, NT>* slots) {
   }
 }
 
-template <typename T, int NT>
+template <typename T, int NT, int W>
 __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, unsigned char* smem, int yp) {
-  constexpr int TP = BShape<NT>::TP, LPT = BShape<NT>::LPT, TPW = BShape<NT>::TPW;
+  constexpr int TP = BShape<NT, W>::TP, LPT = BShape<NT, W>::LPT, TPW = BShape<NT, W>::TPW;
   BSlot<T, NT>* slots = reinterpret_cast<BSlot<T, NT>*>(smem);
   const int slot = threadIdx.x / LPT, q = threadIdx.x % LPT;
   BSlot<T, NT>& S = slots[slot];
@@ -204,7 +225,6 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
   const bool fwd = op == SOCU_OP_FACTOR_AND_SOLVE || op == SOCU_OP_SOLVE || op == SOCU_OP_FORWARD;
   const bool bwd = op == SOCU_OP_FACTOR_AND_SOLVE || op == SOCU_OP_SOLVE || op == SOCU_OP_BACKWARD;
   const int Lv = recursive_iterations(N);
-  const int kp = (NT == 8) ? n : ((n + 3) & ~3);  // inner extent of tile products
   T* At = S.A;
   T* Bt = S.B;
   T* Ct = S.C;
@@ -245,43 +265,44 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
         T* yk = (fwd && task) ? x + (size_t)k * nm : nullptr;
 
         // ---- P1: loads ----
-        tload<T, NT>(At, Lk, n, q, true);
+        tload<T, NT, W>(At, Lk, n, q, true);
         if (up) {
-          tload<T, NT>(Bt, Rl, n, q, false);
-          tload<T, NT>(Ct, Lcr, n, q, false);
-          if (fac) tload<T, NT>(Dt, Lcl, n, q, false);
+          tload<T, NT, W>(Bt, Rl, n, q, false);
+          tload<T, NT, W>(Ct, Lcr, n, q, false);
+          if (fac) tload<T, NT, W>(Dt, Lcl, n, q, false);
         } else if (node && fac) {  // level 0: the original couplings of node k
-          tload<T, NT>(Bt, Rk, n, q, false);
-          tload<T, NT>(Dt, Lck, n, q, false);
+          tload<T, NT, W>(Bt, Rk, n, q, false);
+          tload<T, NT, W>(Dt, Lck, n, q, false);
         }
         if (fwd) {
-          yload<T, NT>(Y0, yp, yk, n, m, q);
+          yload<T, NT, W>(Y0, yp, yk, n, m, q);
           if (up) {
-            yload<T, NT>(Y1, yp, x + (size_t)(k - h) * nm, n, m, q);
-            yload<T, NT>(Y2, yp, right ? x + (size_t)(k + h) * nm : nullptr, n, m, q);
+            yload<T, NT, W>(Y1, yp, x + (size_t)(k - h) * nm, n, m, q);
+            yload<T, NT, W>(Y2, yp, right ? x + (size_t)(k + h) * nm : nullptr, n, m, q);
           }
         }
         if (!fac && node && q < NT) dv[q] = (q < n) ? T(1) / Lk[(size_t)q * n + q] : T(1);
         __syncthreads();
 
         // ---- P2: updates from level l-1 ----
-        T keep[4];
-        int ki[4], kj[4];
+        constexpr int EPL = BShape<NT, W>::EPL;  // product outputs per lane
+        T keep[EPL];
+        int ki[EPL], kj[EPL];
         int nk = 0;
         if (up) {
           if (fac)
-            tprod<T, NT>(
-                kp, [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Bt[j * TP + kk]; },
-                right, kp, [&](int i, int kk) { return Ct[kk * TP + i]; },
+            tprod<T, NT, W>(
+                [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Bt[j * TP + kk]; },
+                right, [&](int i, int kk) { return Ct[kk * TP + i]; },
                 [&](int kk, int j) { return Ct[kk * TP + j]; }, [&](int i, int j, T v) { At[i * TP + j] -= v; });
           if (fwd)
-            vprod<T, NT>(
+            vprod<T, NT, W>(
                 n, m, n, [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Y1[kk * yp + j]; },
                 right, n, [&](int i, int kk) { return Ct[kk * TP + i]; },
                 [&](int kk, int j) { return Y2[kk * yp + j]; }, [&](int i, int j, T v) { Y0[i * yp + j] -= v; });
           if (fac && has_lc)
-            tprod1<T, NT>(
-                kp, [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Dt[kk * TP + j]; },
+            tprod1<T, NT, W>(
+                [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Dt[kk * TP + j]; },
                 [&](int i, int j, T v) {
                   keep[nk] = (i < n && j < n) ? Lck[(size_t)i * n + j] - v : T(0);
                   ki[nk] = i;
@@ -294,7 +315,7 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
         // ---- P3: finish the fill-in of Lc_k, stage R_{k+h}; deferred stores ----
         if (up && fac && has_lc)
           for (int e = 0; e < nk; ++e) Dt[ki[e] * TP + kj[e]] = keep[e];
-        if (up && fac) tload<T, NT>(Bt, Rr, n, q, false);
+        if (up && fac) tload<T, NT, W>(Bt, Rr, n, q, false);
         if (defer) {
           if (fac)
             for (int idx = q; idx < n * n; idx += LPT) Lk[idx] = At[(idx / n) * TP + idx % n];
@@ -307,8 +328,8 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
         if (l > 0 && fac) {
           nk = 0;
           if (has_r)
-            tprod1<T, NT>(
-                kp, [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Ct[kk * TP + j]; },
+            tprod1<T, NT, W>(
+                [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Ct[kk * TP + j]; },
                 [&](int i, int j, T v) {
                   keep[nk] = (i < n && j < n) ? Rk[(size_t)i * n + j] - v : T(0);
                   ki[nk] = i;
@@ -322,7 +343,7 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
 
         // ---- P5: factor and invert the diagonal blocks ----
         if (fac) {
-          slots_chol<T, NT>(slots);
+          slots_chol<T, NT, W>(slots);
           __syncthreads();
         }
 
@@ -331,20 +352,20 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
           if (fac) {
             for (int idx = q; idx < n * n; idx += LPT) Lk[idx] = At[(idx / n) * TP + idx % n];
             if (has_lc)  // Lc_k <- Linv_k Lc_k
-              tprod1<T, NT>(
-                  kp, linv, [&](int kk, int j) { return Dt[kk * TP + j]; },
+              tprod1<T, NT, W>(
+                  linv, [&](int kk, int j) { return Dt[kk * TP + j]; },
                   [&](int i, int j, T v) {
                     if (i < n && j < n) Lck[(size_t)i * n + j] = v;
                   });
             if (has_r)  // R_k <- R_k Linv_k^T
-              tprod1<T, NT>(
-                  kp, [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return linv(j, kk); },
+              tprod1<T, NT, W>(
+                  [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return linv(j, kk); },
                   [&](int i, int j, T v) {
                     if (i < n && j < n) Rk[(size_t)i * n + j] = v;
                   });
           }
           if (fwd)  // y_k <- Linv_k y_k
-            vprod<T, NT>(
+            vprod<T, NT, W>(
                 n, m, n, linv, [&](int kk, int j) { return Y0[kk * yp + j]; }, false, 0, linv,
                 [&](int kk, int j) { return Y0[kk * yp + j]; }, [&](int i, int j, T v) { yk[(size_t)i * m + j] = v; });
         }
@@ -366,34 +387,25 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
         const bool right = node && i + s < N, left = node && i >= s;
         T* Li = node ? L + (size_t)i * nn : nullptr;
         T* xi = node ? x + (size_t)i * nm : nullptr;
-        tload<T, NT>(At, Li, n, q, true);
-        tload<T, NT>(Bt, right ? E + (size_t)(off + i / s) * nn : nullptr, n, q, false);       // R_i
-        tload<T, NT>(Ct, left ? E + (size_t)(off + (i - s) / s) * nn : nullptr, n, q, false);  // Lc_i
-        yload<T, NT>(Y0, yp, xi, n, m, q);
-        yload<T, NT>(Y1, yp, right ? x + (size_t)(i + s) * nm : nullptr, n, m, q);
-        yload<T, NT>(Y2, yp, left ? x + (size_t)(i - s) * nm : nullptr, n, m, q);
+        tload<T, NT, W>(At, Li, n, q, true);
+        tload<T, NT, W>(Bt, right ? E + (size_t)(off + i / s) * nn : nullptr, n, q, false);       // R_i
+        tload<T, NT, W>(Ct, left ? E + (size_t)(off + (i - s) / s) * nn : nullptr, n, q, false);  // Lc_i
+        yload<T, NT, W>(Y0, yp, xi, n, m, q);
+        yload<T, NT, W>(Y1, yp, right ? x + (size_t)(i + s) * nm : nullptr, n, m, q);
+        yload<T, NT, W>(Y2, yp, left ? x + (size_t)(i - s) * nm : nullptr, n, m, q);
         if (node && q < NT) dv[q] = (q < n) ? T(1) / Li[(size_t)q * n + q] : T(1);
         __syncthreads();
-        // x_i - R_i^T x_{i+s} - Lc_i x_{i-s}
-        T keep[4];
-        int ki[4], kj[4];
-        int nk = 0;
+        // x_i - R_i^T x_{i+s} - Lc_i x_{i-s} (in place: element (r, j) of Y0 is
+        // read and written by one lane only)
         if (node)
-          vprod<T, NT>(
+          vprod<T, NT, W>(
               n, m, n, [&](int r, int kk) { return Bt[kk * TP + r]; }, [&](int kk, int j) { return Y1[kk * yp + j]; },
               true, n, [&](int r, int kk) { return Ct[r * TP + kk]; }, [&](int kk, int j) { return Y2[kk * yp + j]; },
-              [&](int r, int j, T v) {
-                keep[nk] = Y0[r * yp + j] - v;
-                ki[nk] = r;
-                kj[nk] = j;
-                ++nk;
-              });
-        __syncthreads();
-        for (int e = 0; e < nk; ++e) Y0[ki[e] * yp + kj[e]] = keep[e];
+              [&](int r, int j, T v) { Y0[r * yp + j] -= v; });
         __syncthreads();
         // x_i <- Linv_i^T x_i
         if (node)
-          vprod<T, NT>(
+          vprod<T, NT, W>(
               n, m, n, [&](int r, int kk) { return linv(kk, r); }, [&](int kk, int j) { return Y0[kk * yp + j]; },
               false, 0, [&](int r, int kk) { return linv(kk, r); }, [&](int kk, int j) { return Y0[kk * yp + j]; },
               [&](int r, int j, T v) { xi[(size_t)r * m + j] = v; });
@@ -404,33 +416,45 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
 }
 
 // One workgroup per system of tile class NT; `order` lists the systems.
-template <typename T, int NT>
-__global__ __launch_bounds__(kBT) void k_batched(const BatchProblem* __restrict__ probs, const int* __restrict__ order,
-                                                 const uint8_t* __restrict__ mask, int op, int yp) {
+template <typename T, int NT, int W>
+__global__ __launch_bounds__(64 * W) void k_batched(const BatchProblem* __restrict__ probs,
+                                                    const int* __restrict__ order, const uint8_t* __restrict__ mask,
+                                                    int op, int yp) {
   const int p = order[blockIdx.x];
   if (masked_out(mask, p)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const BatchProblem P = probs[p];
-  batched_system<T, NT>(P, op, smem_raw, yp);
+  batched_system<T, NT, W>(P, op, smem_raw, yp);
 }
 
-template <typename T, int NT>
+template <typename T, int NT, int W>
 size_t lds_bytes(int yp) {
-  return BShape<NT>::TPW * (sizeof(BSlot<T, NT>) + (size_t)3 * NT * yp * sizeof(T));
+  return BShape<NT, W>::TPW * (sizeof(BSlot<T, NT>) + (size_t)3 * NT * yp * sizeof(T));
 }
 
-template <typename T, int NT>
+template <typename T, int NT, int W>
 hipError_t set_attr() {
-  return hipFuncSetAttribute((const void*)k_batched<T, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)lds_bytes<T, NT>(NT + 1));
+  return hipFuncSetAttribute((const void*)k_batched<T, NT, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds_bytes<T, NT, W>(NT + 1));
 }
 
 template <typename T, int NT>
-void launch_nt(const BatchProblem* dprobs, const int* dorder, int count, int mmax, const uint8_t* mask, int op,
-               hipStream_t st) {
+hipError_t set_attrs() {
+  hipError_t e = set_attr<T, NT, 1>();
+  return e == hipSuccess ? set_attr<T, NT, 4>() : e;
+}
+
+template <typename T, int NT>
+void launch_nt(const BatchProblem* dprobs, const int* dorder, int count, int mmax, int waves, const uint8_t* mask,
+               int op, hipStream_t st) {
   const int yp = mmax + 1;
-  const size_t lds = lds_bytes<T, NT>(yp);
-  hipLaunchKernelGGL((k_batched<T, NT>), dim3(count), dim3(kBT), lds, st, dprobs, dorder, mask, op, yp);
+  if (waves == 1) {
+    const size_t lds = lds_bytes<T, NT, 1>(yp);
+    hipLaunchKernelGGL((k_batched<T, NT, 1>), dim3(count), dim3(64), lds, st, dprobs, dorder, mask, op, yp);
+  } else {
+    const size_t lds = lds_bytes<T, NT, 4>(yp);
+    hipLaunchKernelGGL((k_batched<T, NT, 4>), dim3(count), dim3(256), lds, st, dprobs, dorder, mask, op, yp);
+  }
 }
 
 // Copies every system's L, E and x from src to dst (same shapes); one
@@ -450,29 +474,29 @@ __global__ __launch_bounds__(256) void k_copy_systems(const BatchProblem* __rest
 }  // namespace
 
 hipError_t init_batched_kernels() {
-  hipError_t e = set_attr<double, 8>();
-  if (e == hipSuccess) e = set_attr<double, 16>();
-  if (e == hipSuccess) e = set_attr<double, 32>();
-  if (e == hipSuccess) e = set_attr<float, 8>();
-  if (e == hipSuccess) e = set_attr<float, 16>();
-  if (e == hipSuccess) e = set_attr<float, 32>();
+  hipError_t e = set_attrs<double, 8>();
+  if (e == hipSuccess) e = set_attrs<double, 16>();
+  if (e == hipSuccess) e = set_attrs<double, 32>();
+  if (e == hipSuccess) e = set_attrs<float, 8>();
+  if (e == hipSuccess) e = set_attrs<float, 16>();
+  if (e == hipSuccess) e = set_attrs<float, 32>();
   return e;
 }
 
 int batched_class(int n) { return n <= 8 ? 0 : (n <= 16 ? 1 : 2); }
 
-hipError_t batched_launch_class(const BatchProblem* dprobs, const int* dorder, int cls, int count, int mmax,
+hipError_t batched_launch_class(const BatchProblem* dprobs, const int* dorder, int cls, int count, int mmax, int waves,
                                 const uint8_t* mask, int op, int dtype, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  if (cls < 0 || cls > 2 || mmax < 1 || mmax > (8 << cls)) return hipErrorInvalidValue;
+  if (cls < 0 || cls > 2 || mmax < 1 || mmax > (8 << cls) || (waves != 1 && waves != 4)) return hipErrorInvalidValue;
   if (dtype == SOCU_DTYPE_F64) {
-    if (cls == 0) launch_nt<double, 8>(dprobs, dorder, count, mmax, mask, op, st);
-    else if (cls == 1) launch_nt<double, 16>(dprobs, dorder, count, mmax, mask, op, st);
-    else launch_nt<double, 32>(dprobs, dorder, count, mmax, mask, op, st);
+    if (cls == 0) launch_nt<double, 8>(dprobs, dorder, count, mmax, waves, mask, op, st);
+    else if (cls == 1) launch_nt<double, 16>(dprobs, dorder, count, mmax, waves, mask, op, st);
+    else launch_nt<double, 32>(dprobs, dorder, count, mmax, waves, mask, op, st);
   } else {
-    if (cls == 0) launch_nt<float, 8>(dprobs, dorder, count, mmax, mask, op, st);
-    else if (cls == 1) launch_nt<float, 16>(dprobs, dorder, count, mmax, mask, op, st);
-    else launch_nt<float, 32>(dprobs, dorder, count, mmax, mask, op, st);
+    if (cls == 0) launch_nt<float, 8>(dprobs, dorder, count, mmax, waves, mask, op, st);
+    else if (cls == 1) launch_nt<float, 16>(dprobs, dorder, count, mmax, waves, mask, op, st);
+    else launch_nt<float, 32>(dprobs, dorder, count, mmax, waves, mask, op, st);
   }
   return hipGetLastError();
 }

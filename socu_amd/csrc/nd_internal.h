@@ -78,9 +78,10 @@ struct BatchProblem {
 hipError_t init_batched_kernels();
 // Tile class of a block size: 0 (tiles of 8), 1 (16), 2 (32).
 int batched_class(int n);
-// One launch over `count` systems with tile class `cls`, listed (by index
-// into dprobs) at dorder; mmax is the largest m among them (<= 8 << cls).
-hipError_t batched_launch_class(const BatchProblem* dprobs, const int* dorder, int cls, int count, int mmax,
+// One launch over `count` systems with tile class `cls` and workgroups of
+// `waves` (1 or 4) wavefronts, listed (by index into dprobs) at dorder; mmax
+// is the largest m among them (<= 8 << cls).
+hipError_t batched_launch_class(const BatchProblem* dprobs, const int* dorder, int cls, int count, int mmax, int waves,
                                 const uint8_t* mask, int op, int dtype, hipStream_t st);
 // Copies L, E (and x unless op is a factor) of count systems src[k] -> dst[k].
 hipError_t batched_copy(const BatchProblem* dsrc, const BatchProblem* ddst, int count, int op, int dtype,

@@ -179,6 +179,95 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
   if (rowlane) dinv[r] = rinv;
 }
 
+// Cholesky and inverse of a 16x16 tile by lanes 0-31 of the wavefront (row
+// lanes 0-15, inverse lanes 16-31; lanes 32-63 must not call it), with the
+// recurrence and in/out convention of tile_chol_inv.
+template <typename T>
+__device__ __forceinline__ void tile_chol_inv16(T* S, int pitch, T* dinv, T* col) {
+  const int lane = lane_id();
+  const int r = lane & 15;
+  const bool rowlane = lane < 16;
+  T u[16];
+  T rinv = T(1);
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    const T v = S[r * pitch + c];
+    u[c] = rowlane ? ((c <= r) ? v : T(0)) : ((c == r) ? T(1) : T(0));
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    if (rowlane) col[r] = u[k];
+    const T p = bcast_lane(u[k], k);
+    const T inv = rsqrt_t(p);
+    const T lk = rowlane ? ((r > k) ? u[k] * inv : T(0)) : u[k] * inv;
+    const T coef = lk * inv;
+#pragma unroll
+    for (int q = k + 1; q < 16; ++q) u[q] -= coef * col[q];
+    u[k] = rowlane ? ((r == k) ? p * inv : ((r > k) ? lk : u[k])) : lk;
+    rinv = (r == k) ? inv : rinv;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) asm volatile("" : "+v"(u[q]));
+  }
+#pragma unroll
+  for (int c = 0; c < 16; ++c)
+    if (rowlane ? (c <= r) : (c > r)) S[r * pitch + c] = u[c];
+  if (rowlane) dinv[r] = rinv;
+}
+
+// One 16x16 MFMA product over K = 16: acc = A B with fa(i, k), fb(k, j).
+template <typename T, typename FA, typename FB>
+__device__ __forceinline__ typename Mfma<T>::acc_t mfma16x16x16(FA fa, FB fb) {
+  const int l = lane_id();
+  const int i = l & 15, kq = l >> 4;
+  typename Mfma<T>::acc_t acc = acc_zero<T>();
+#pragma unroll
+  for (int k = 0; k < 16; k += 4) acc = Mfma<T>::mma(fa(i, k + kq), fb(k + kq, i), acc);
+  return acc;
+}
+
+// 32x32 Cholesky + inverse by one wavefront through 16x16 halves:
+// A11 = L11 L11^T; L21 = A21 L11^-T; A22 - L21 L21^T = L22 L22^T;
+// Linv21 = -Linv22 L21 Linv11.  Same in/out convention as tile_chol_inv.
+template <typename T>
+__device__ __forceinline__ void tile_chol_inv32b(T* S, int pitch, T* dinv, T* col) {
+  const int l = lane_id();
+  T* S11 = S;
+  T* S21 = S + 16 * pitch;
+  T* S22 = S + 16 * pitch + 16;
+  T* S12 = S + 16;
+  if (l < 32) tile_chol_inv16<T>(S11, pitch, dinv, col);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // Linv11[a][b] (a >= b) from the factored S11
+  auto linv11 = [&](int a, int b) -> T { return (b < a) ? S11[b * pitch + a] : ((a == b) ? dinv[a] : T(0)); };
+  // L21 = A21 Linv11^T
+  auto acc = mfma16x16x16<T>([&](int i, int k) { return S21[i * pitch + k]; },
+                             [&](int k, int j) { return linv11(j, k); });
+  acc16_foreach<T>(acc, [&](int i, int j, T v) { S21[i * pitch + j] = v; });
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // A22 -= L21 L21^T
+  acc = mfma16x16x16<T>([&](int i, int k) { return S21[i * pitch + k]; },
+                        [&](int k, int j) { return S21[j * pitch + k]; });
+  acc16_foreach<T>(acc, [&](int i, int j, T v) { S22[i * pitch + j] -= v; });
+  // T1 = L21 Linv11 (independent of the A22 factor)
+  auto t1 = mfma16x16x16<T>([&](int i, int k) { return S21[i * pitch + k]; },
+                            [&](int k, int j) { return linv11(k, j); });
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (l < 32) tile_chol_inv16<T>(S22, pitch, dinv + 16, col);
+  // park T1 in the upper-right block (row-major), then Linv21 = -Linv22 T1
+  acc16_foreach<T>(t1, [&](int i, int j, T v) { S12[i * pitch + j] = v; });
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  auto linv22 = [&](int a, int b) -> T { return (b < a) ? S22[b * pitch + a] : ((a == b) ? dinv[16 + a] : T(0)); };
+  acc = mfma16x16x16<T>([&](int i, int k) { return linv22(i, k); }, [&](int k, int j) { return S12[k * pitch + j]; });
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // Linv21[i][j] stored transposed: S[j][16 + i]
+  acc16_foreach<T>(acc, [&](int i, int j, T v) { S12[j * pitch + i] = -v; });
+}
+
 // Cholesky and inverse of G = 64/(2*NT) independent NT x NT tiles per
 // wavefront (NT = 8, 16 or 32): lanes [g*2NT, g*2NT+NT) keep the rows of tile
 // g, lanes [g*2NT+NT, (g+1)*2NT) its inverse columns, with the same recurrence

@@ -69,7 +69,7 @@ def test_uneven_factor_and_solve_matches_oracle(tune, graph, device):
         assert nd.rel_residual(L0[None], E0[None], xs[k].cpu().numpy()[None], b0[None])[0] < 1e-12
     cfg = run.config()
     assert sum(v["systems"] for v in cfg.values()) == len(SHAPES)
-    assert all(v["tile"] in (8, 16, 32) and v["tile"] >= bound for bound, v in cfg.items())
+    assert all(v["tile"] in (8, 16, 32) and v["tile"] >= bound and v["waves"] in (1, 4) for bound, v in cfg.items())
 
 
 def test_graph_replays_match_eager(device):

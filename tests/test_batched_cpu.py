@@ -81,8 +81,9 @@ def test_c_abi_empty_batch_needs_no_device():
     rc, h = _create([])
     assert rc == _lib.OK
     counts = (ctypes.c_int * 3)()
+    tiles = (ctypes.c_int * 3)()
     waves = (ctypes.c_int * 3)()
-    assert lib.socu_batched_config(h, counts, waves) == _lib.OK
+    assert lib.socu_batched_config(h, counts, tiles, waves) == _lib.OK
     assert list(counts) == [0, 0, 0]
     assert lib.socu_batched_execute(h, None) == _lib.OK
     assert lib.socu_batched_destroy(h) == _lib.OK
