@@ -73,19 +73,34 @@ __global__ __launch_bounds__(kThreads) void k_refine_init(double* __restrict__ x
   if (threadIdx.x == 0) partial[(size_t)b * nblk + blockIdx.x] = red[0];
 }
 
-// state[b] <- (||b||^2, ||b||^2, done = masked out, 0); one thread per b.
-__global__ void k_refine_start(double* __restrict__ state, uint8_t* __restrict__ active,
-                               const double* __restrict__ partial, int nblk, int B, const uint8_t* __restrict__ mask) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
+// Sum of partial[b][0..nblk) by one workgroup (fixed order: deterministic).
+__device__ __forceinline__ double block_sum(const double* __restrict__ part, int nblk, double* red) {
   double s = 0.0;
-  for (int q = 0; q < nblk; ++q) s += partial[(size_t)b * nblk + q];
-  const bool off = masked_out(mask, b);
-  state[4 * b + 0] = s;
-  state[4 * b + 1] = s;
-  state[4 * b + 2] = off ? 1.0 : 0.0;
-  state[4 * b + 3] = 0.0;
-  active[b] = off ? 0 : 1;
+  for (int q = threadIdx.x; q < nblk; q += kThreads) s += part[q];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+// state[b] <- (||b||^2, ||b||^2, done = masked out, 0); one workgroup per b.
+__global__ __launch_bounds__(kThreads) void k_refine_start(double* __restrict__ state, uint8_t* __restrict__ active,
+                                                          const double* __restrict__ partial, int nblk,
+                                                          const uint8_t* __restrict__ mask) {
+  const int b = blockIdx.x;
+  __shared__ double red[kThreads];
+  const double s = block_sum(partial + (size_t)b * nblk, nblk, red);
+  if (threadIdx.x == 0) {
+    const bool off = masked_out(mask, b);
+    state[4 * b + 0] = s;
+    state[4 * b + 1] = s;
+    state[4 * b + 2] = off ? 1.0 : 0.0;
+    state[4 * b + 3] = 0.0;
+    active[b] = off ? 0 : 1;
+  }
 }
 
 // x += d (FP32 correction, converted), for systems not yet converged.
@@ -97,27 +112,31 @@ __global__ __launch_bounds__(kThreads) void k_refine_update(double* __restrict__
     x[(size_t)b * count + i] += (double)d[(size_t)b * count + i];
 }
 
-// r = b - Psi x for block row i (FP64), one workgroup per (row, batch):
+// r = b - Psi x (FP64) for a slice of kResRows rows of block row i, one
+// workgroup per (block row, slice, batch):
 //   r_i = b_i - sym(L_i) x_i - E_{i-1} x_{i-1} - E_i^T x_{i+1}
 // Only tril(L_i) is read (the matrix the reference's solver sees).
 // Row-access products (tril(L_i) x_i, E_{i-1} x_{i-1}) use 8 lanes per row
 // and a shuffle reduction; column-access products (strict upper of sym(L_i)
-// from the lower triangle, E_i^T x_{i+1}) use one thread per row and K parts.
-// The FP32 copy of r feeds the next correction solve; partial[b][i] gets
-// ||r_i||^2.
-constexpr int kResM = 16;  // right-hand sides per residual workgroup pass
-
+// from the lower triangle, E_i^T x_{i+1}) use 8 groups of 32 lanes, lane r of
+// a group owning output row r, so that each load instruction reads 32
+// consecutive entries of one matrix row, and the 8 partial sums meet in LDS.
+// The FP32 copy of r feeds the next correction solve; partial[b][blockIdx.x]
+// gets the slice's ||r||^2.
+constexpr int kResM = 8;      // right-hand sides per pass
+constexpr int kResRows = 32;  // output rows per workgroup
 __global__ __launch_bounds__(kThreads) void k_residual(const double* __restrict__ L, const double* __restrict__ E,
                                                       const double* __restrict__ x, const double* __restrict__ b_in,
                                                       float* __restrict__ r32, double* __restrict__ partial,
-                                                      const double* __restrict__ state, int N, int n, int m, int nE) {
+                                                      const double* __restrict__ state, int N, int n, int m, int nE,
+                                                      int nsl) {
   const int b = blockIdx.y;
-  const int i = blockIdx.x;
+  const int i = blockIdx.x / nsl, r0 = (blockIdx.x - i * nsl) * kResRows;
   if (refine_done(state, b)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const size_t nn = (size_t)n * n, nm = (size_t)n * m;
   double* xs = reinterpret_cast<double*>(smem_raw);  // 3 x n x m: x_{i-1}, x_i, x_{i+1}
-  double* acc_s = xs + 3 * nm;                        // n x m accumulated Psi x
+  __shared__ double part[8][kResRows][kResM];
   __shared__ double red[kThreads];
   const double* xb = x + (size_t)b * N * nm;
   for (int q = threadIdx.x; q < 3 * (int)nm; q += kThreads) {
@@ -125,69 +144,72 @@ __global__ __launch_bounds__(kThreads) void k_residual(const double* __restrict_
     const int row = i - 1 + which;
     xs[q] = (row >= 0 && row < N) ? xb[(size_t)row * nm + e] : 0.0;
   }
-  for (int q = threadIdx.x; q < (int)nm; q += kThreads) acc_s[q] = 0.0;
   __syncthreads();
   const double* Li = L + ((size_t)b * N + i) * nn;
   const double* Eb = E + (size_t)b * nE * nn;
-  const double* Em = (i > 0) ? Eb + (size_t)(i - 1) * nn : nullptr;     // block (i, i-1)
-  const double* Ep = (i + 1 < N) ? Eb + (size_t)i * nn : nullptr;       // block (i+1, i)
+  const double* Em = (i > 0) ? Eb + (size_t)(i - 1) * nn : nullptr;  // block (i, i-1)
+  const double* Ep = (i + 1 < N) ? Eb + (size_t)i * nn : nullptr;    // block (i+1, i)
   const double* x_prev = xs;
   const double* x_cur = xs + nm;
   const double* x_next = xs + 2 * nm;
+  const double* bi = b_in + (size_t)b * N * nm + (size_t)i * nm;
+  float* ri = r32 + (size_t)b * N * nm + (size_t)i * nm;
   const int tid = threadIdx.x;
-  // (1) row products: 8 lanes per row
-  {
-    const int sub = tid & 7;
-    for (int r = tid >> 3; r < n; r += kThreads / 8) {
-      for (int c0 = 0; c0 < m; c0 += kResM) {
-        const int w = min(kResM, m - c0);
-        double acc[kResM];
+  const int cr = tid & 31, cg = tid >> 5;  // column products: row r0 + cr, k = cg (mod 8)
+  const int rr = tid >> 3, rs = tid & 7;   // row products: row r0 + rr, k = rs (mod 8)
+  double sq = 0.0;
+  for (int c0 = 0; c0 < m; c0 += kResM) {
+    const int w = min(kResM, m - c0);
+    {
+      const int r = r0 + cr;
+      double acc[kResM];
 #pragma unroll
-        for (int c = 0; c < kResM; ++c) acc[c] = 0.0;
-        for (int k = sub; k < n; k += 8) {
+      for (int c = 0; c < kResM; ++c) acc[c] = 0.0;
+      if (r < n) {
+#pragma unroll 4
+        for (int k = cg; k < n; k += 8) {
+          const double lv = (k > r) ? Li[(size_t)k * n + r] : 0.0;
+          const double ev = Ep ? Ep[(size_t)k * n + r] : 0.0;
+#pragma unroll
+          for (int c = 0; c < kResM; ++c)
+            if (c < w) acc[c] += lv * x_cur[(size_t)k * m + c0 + c] + ev * x_next[(size_t)k * m + c0 + c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < kResM; ++c) part[cg][cr][c] = acc[c];
+    }
+    double racc[kResM];
+#pragma unroll
+    for (int c = 0; c < kResM; ++c) racc[c] = 0.0;
+    {
+      const int r = r0 + rr;
+      if (r < n) {
+#pragma unroll 4
+        for (int k = rs; k < n; k += 8) {
           const double lv = (k <= r) ? Li[(size_t)r * n + k] : 0.0;
           const double ev = Em ? Em[(size_t)r * n + k] : 0.0;
 #pragma unroll
           for (int c = 0; c < kResM; ++c)
-            if (c < w) acc[c] += lv * x_cur[(size_t)k * m + c0 + c] + ev * x_prev[(size_t)k * m + c0 + c];
+            if (c < w) racc[c] += lv * x_cur[(size_t)k * m + c0 + c] + ev * x_prev[(size_t)k * m + c0 + c];
         }
-#pragma unroll
-        for (int c = 0; c < kResM; ++c) {
-          double v = acc[c];
-          for (int o = 4; o >= 1; o >>= 1) v += __shfl_xor(v, o, 8);
-          if (sub == 0 && c < w) acc_s[(size_t)r * m + c0 + c] += v;
-        }
-      }
-    }
-  }
-  __syncthreads();
-  // (2) column products: thread per row r, loop over k (reads coalesced in r)
-  for (int r = tid; r < n; r += kThreads) {
-    for (int c0 = 0; c0 < m; c0 += kResM) {
-      const int w = min(kResM, m - c0);
-      double acc[kResM];
-#pragma unroll
-      for (int c = 0; c < kResM; ++c) acc[c] = 0.0;
-      for (int k = 0; k < n; ++k) {
-        const double lv = (k > r) ? Li[(size_t)k * n + r] : 0.0;
-        const double ev = Ep ? Ep[(size_t)k * n + r] : 0.0;
-#pragma unroll
-        for (int c = 0; c < kResM; ++c)
-          if (c < w) acc[c] += lv * x_cur[(size_t)k * m + c0 + c] + ev * x_next[(size_t)k * m + c0 + c];
       }
 #pragma unroll
       for (int c = 0; c < kResM; ++c)
-        if (c < w) acc_s[(size_t)r * m + c0 + c] += acc[c];
+        for (int o = 4; o >= 1; o >>= 1) racc[c] += __shfl_xor(racc[c], o, 8);
     }
-  }
-  __syncthreads();
-  double sq = 0.0;
-  const double* bi = b_in + (size_t)b * N * nm + (size_t)i * nm;
-  float* ri = r32 + (size_t)b * N * nm + (size_t)i * nm;
-  for (int q = tid; q < (int)nm; q += kThreads) {
-    const double v = bi[q] - acc_s[q];
-    ri[q] = (float)v;
-    sq += v * v;
+    __syncthreads();
+    if (rs == 0 && r0 + rr < n) {
+      const int r = r0 + rr;
+      for (int c = 0; c < w; ++c) {
+        double v = racc[c];
+#pragma unroll
+        for (int g = 0; g < 8; ++g) v += part[g][rr][c];
+        const double res = bi[(size_t)r * m + c0 + c] - v;
+        ri[(size_t)r * m + c0 + c] = (float)res;
+        sq += res * res;
+      }
+    }
+    __syncthreads();
   }
   red[tid] = sq;
   __syncthreads();
@@ -195,22 +217,24 @@ __global__ __launch_bounds__(kThreads) void k_residual(const double* __restrict_
     if (tid < o) red[tid] += red[tid + o];
     __syncthreads();
   }
-  if (tid == 0) partial[(size_t)b * N + i] = red[0];
+  if (tid == 0) partial[(size_t)b * gridDim.x + blockIdx.x] = red[0];
 }
 
 // ||r||^2 from the partials; sets the done flag when ||r|| <= tol ||b||.
-__global__ void k_refine_check(double* __restrict__ state, uint8_t* __restrict__ active,
-                               const double* __restrict__ partial, int nblk, int B, double tol2) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
+// One workgroup per system.
+__global__ __launch_bounds__(kThreads) void k_refine_check(double* __restrict__ state, uint8_t* __restrict__ active,
+                                                          const double* __restrict__ partial, int nblk, double tol2) {
+  const int b = blockIdx.x;
   if (state[4 * b + 2] != 0.0) return;
-  double s = 0.0;
-  for (int q = 0; q < nblk; ++q) s += partial[(size_t)b * nblk + q];
-  state[4 * b + 1] = s;
-  state[4 * b + 3] += 1.0;
-  if (s <= tol2 * state[4 * b + 0] || !(s == s)) {  // converged, or NaN (non-SPD input)
-    state[4 * b + 2] = 1.0;
-    active[b] = 0;
+  __shared__ double red[kThreads];
+  const double s = block_sum(partial + (size_t)b * nblk, nblk, red);
+  if (threadIdx.x == 0) {
+    state[4 * b + 1] = s;
+    state[4 * b + 3] += 1.0;
+    if (s <= tol2 * state[4 * b + 0] || !(s == s)) {  // converged, or NaN (non-SPD input)
+      state[4 * b + 2] = 1.0;
+      active[b] = 0;
+    }
   }
 }
 
@@ -226,11 +250,13 @@ static int refine_blocks(size_t count) {
   return (int)(blk < 256 ? (blk > 0 ? blk : 1) : 256);
 }
 
-size_t mixed_residual_lds(int n, int m) { return sizeof(double) * 4 * (size_t)n * m; }
+size_t mixed_residual_lds(int n, int m) { return sizeof(double) * 3 * (size_t)n * m; }
 
-// Dynamic LDS cap of the residual kernel (its static LDS is 2 KiB of the
+// Dynamic LDS cap of the residual kernel (its static LDS is 18 KiB of the
 // 160 KiB per workgroup).
-constexpr size_t kResidualLdsMax = 144 * 1024;
+constexpr size_t kResidualLdsMax = 140 * 1024;
+
+static int residual_slices(int n) { return (n + kResRows - 1) / kResRows; }
 
 hipError_t init_refine_kernels() {
   return hipFuncSetAttribute((const void*)k_residual, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -249,7 +275,8 @@ hipError_t mixed_alloc(MixedPlan& p) {
   if ((e = hipMalloc(&p.d32, sizeof(float) * p.B * (size_t)p.N * nm)) != hipSuccess) return e;
   if ((e = hipMalloc(&p.b64, sizeof(double) * p.B * (size_t)p.N * nm)) != hipSuccess) return e;
   if ((e = hipMalloc(&p.state, sizeof(double) * 4 * p.B)) != hipSuccess) return e;
-  const int np = p.N > p.nblk_init ? p.N : p.nblk_init;
+  const int nres = p.N * residual_slices(p.n);
+  const int np = nres > p.nblk_init ? nres : p.nblk_init;
   if ((e = hipMalloc(&p.partial, sizeof(double) * p.B * (size_t)np)) != hipSuccess) return e;
   if ((e = hipMalloc(&p.active, (size_t)p.B)) != hipSuccess) return e;
   return hipSuccess;
@@ -287,8 +314,8 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
     if ((e = factor_level_blocked<float>(p.L32, p.E32, p.mask, B, N, n, l, p.f64_pivots, st)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_refine_init, dim3(p.nblk_init, B), dim3(kThreads), 0, st, p.x, p.b64, p.d32, p.partial, xcount,
                      p.nblk_init, p.mask);
-  hipLaunchKernelGGL(k_refine_start, dim3((B + 63) / 64), dim3(64), 0, st, p.state, p.active, p.partial, p.nblk_init,
-                     B, p.mask);
+  hipLaunchKernelGGL(k_refine_start, dim3(B), dim3(kThreads), 0, st, p.state, p.active, p.partial, p.nblk_init,
+                     p.mask);
   const size_t rlds = mixed_residual_lds(n, m);
   for (int it = 0; it < p.max_iter; ++it) {
     for (int step = -1; step < Lv - 1; ++step)
@@ -300,9 +327,10 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
     kt_begin(kKcRefine, st);
     hipLaunchKernelGGL(k_refine_update, dim3(refine_blocks(xcount), B), dim3(kThreads), 0, st, p.x, p.d32, xcount,
                        p.state);
-    hipLaunchKernelGGL(k_residual, dim3(N, B), dim3(kThreads), rlds, st, p.L, p.E, p.x, p.b64, p.d32, p.partial,
-                       p.state, N, n, m, nE);
-    hipLaunchKernelGGL(k_refine_check, dim3((B + 63) / 64), dim3(64), 0, st, p.state, p.active, p.partial, N, B,
+    const int nsl = residual_slices(n);
+    hipLaunchKernelGGL(k_residual, dim3(N * nsl, B), dim3(kThreads), rlds, st, p.L, p.E, p.x, p.b64, p.d32,
+                       p.partial, p.state, N, n, m, nE, nsl);
+    hipLaunchKernelGGL(k_refine_check, dim3(B), dim3(kThreads), 0, st, p.state, p.active, p.partial, N * nsl,
                        p.tol * p.tol);
     kt_end(kKcRefine, st);
   }

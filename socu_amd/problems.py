@@ -142,3 +142,17 @@ def rel_residual(L, E, x, b):
     """||b - Psi x||_F / ||b||_F of one system (arrays shaped as above)."""
     r = b - block_tridiag_matvec(L, E, x)
     return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+def class_bytes(N: int, n: int, m: int, w: int = 4):
+    """Compulsory HBM bytes of one execution of each solve-side kernel class:
+    a forward (or backward) sweep reads tril L and every E block and reads and
+    writes x (w-byte factor scalars); one refinement iteration reads the FP64
+    matrix (tril L and E[:N-1]), x and b, writes the residual (w bytes) and
+    updates x (reads the correction, reads and writes x)."""
+    nE = calculate_off_diag_storage_len(N)
+    tri = N * n * (n + 1) // 2
+    nv = N * n * m
+    sweep = w * (tri + nE * n * n + 2 * nv)
+    refine = 8 * (tri + (N - 1) * n * n + 2 * nv) + w * nv + (w + 16) * nv
+    return {"forward": sweep, "backward": sweep, "refine": refine}
