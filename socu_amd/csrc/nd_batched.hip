@@ -64,20 +64,84 @@ struct BShape {
   static_assert(BT % LPT == 0 && (NT * NT) % LPT == 0 && LPT % NT == 0, "shape");
 };
 
-// Block `src` (n x n, row-major) into tile t; outside n: identity (ident) or 0.
+// Tile loads in two steps so that several tiles' loads are in flight at once:
+// tfetch issues unconditional loads from clamped addresses into registers
+// (nothing when src is NULL); tcommit pins them (after every fetch of the
+// group was issued), masks outside n (identity on the diagonal when ident)
+// and stores the NT x NT tile to LDS.
 template <typename T, int NT, int W>
-__device__ __forceinline__ void tload(T* t, const T* __restrict__ src, int n, int q, bool ident) {
-  constexpr int TP = BShape<NT, W>::TP, LPT = BShape<NT, W>::LPT, EPL = BShape<NT, W>::EPL;
-  T v[EPL];
+__device__ __forceinline__ void tfetch(T* v, const T* __restrict__ src, int n, int q) {
+  constexpr int LPT = BShape<NT, W>::LPT, EPL = BShape<NT, W>::EPL;
+  if (src == nullptr) return;
 #pragma unroll
   for (int e = 0; e < EPL; ++e) {
     const int idx = q + e * LPT, r = idx / NT, c = idx % NT;
-    v[e] = (src != nullptr && r < n && c < n) ? src[(size_t)r * n + c] : ((ident && r == c) ? T(1) : T(0));
+    v[e] = src[(size_t)min(r, n - 1) * n + min(c, n - 1)];
+  }
+}
+
+template <typename T, int NT, int W>
+__device__ __forceinline__ void tcommit(T* t, T* v, const T* src, int n, int q, bool ident) {
+  constexpr int TP = BShape<NT, W>::TP, LPT = BShape<NT, W>::LPT, EPL = BShape<NT, W>::EPL;
+  if (src != nullptr) {
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) pin_reg(v[e]);
   }
 #pragma unroll
   for (int e = 0; e < EPL; ++e) {
     const int idx = q + e * LPT, r = idx / NT, c = idx % NT;
-    t[r * TP + c] = v[e];
+    t[r * TP + c] = (src != nullptr && r < n && c < n) ? v[e] : ((ident && r == c) ? T(1) : T(0));
+  }
+}
+
+template <typename T, int NT, int W>
+__device__ __forceinline__ void tload(T* t, const T* __restrict__ src, int n, int q, bool ident) {
+  T v[BShape<NT, W>::EPL];
+  tfetch<T, NT, W>(v, src, n, q);
+  tcommit<T, NT, W>(t, v, src, n, q, ident);
+}
+
+// Position (i, j) of the e-th output element tprod hands this thread, in the
+// order of its out() calls.
+template <typename T, int NT, int W>
+__device__ __forceinline__ void tprod_pos(int e, int& i, int& j) {
+  if constexpr (NT == 8) {
+    const int q = threadIdx.x & 15;
+    i = 2 * (q >> 2) + (e >> 1);
+    j = 2 * (q & 3) + (e & 1);
+  } else {
+    constexpr int LPT = BShape<NT, W>::LPT;
+    constexpr int TD = NT / 16, WPT = LPT / 64;
+    const int l = lane_id();
+    const int t = (int)(threadIdx.x % LPT) / 64 + (e >> 2) * WPT;
+    i = 16 * (t / TD) + Mfma<T>::row(l, e & 3);
+    j = 16 * (t % TD) + Mfma<T>::col(l, e & 3);
+  }
+}
+
+// Values of block src (n x n) at this thread's tprod output positions
+// (0 outside n); unconditional loads, pinned by the caller's group.
+template <typename T, int NT, int W>
+__device__ __forceinline__ void pfetch(T* v, const T* __restrict__ src, int n) {
+  constexpr int EPL = BShape<NT, W>::EPL;
+  if (src == nullptr) return;
+#pragma unroll
+  for (int e = 0; e < EPL; ++e) {
+    int i, j;
+    tprod_pos<T, NT, W>(e, i, j);
+    v[e] = src[(size_t)min(i, n - 1) * n + min(j, n - 1)];
+  }
+}
+
+template <typename T, int NT, int W>
+__device__ __forceinline__ void pcommit(T* v, const T* src, int n) {
+  constexpr int EPL = BShape<NT, W>::EPL;
+#pragma unroll
+  for (int e = 0; e < EPL; ++e) {
+    int i, j;
+    tprod_pos<T, NT, W>(e, i, j);
+    if (src != nullptr) pin_reg(v[e]);
+    v[e] = (src != nullptr && i < n && j < n) ? v[e] : T(0);
   }
 }
 
@@ -264,16 +328,37 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
         const T* Rr = (up && has_r) ? E + (size_t)(offp + ((k + h) >> sh)) * nn : nullptr;     // (k+s, k+h)
         T* yk = (fwd && task) ? x + (size_t)k * nm : nullptr;
 
-        // ---- P1: loads ----
-        tload<T, NT, W>(At, Lk, n, q, true);
-        if (up) {
-          tload<T, NT, W>(Bt, Rl, n, q, false);
-          tload<T, NT, W>(Ct, Lcr, n, q, false);
-          if (fac) tload<T, NT, W>(Dt, Lcl, n, q, false);
-        } else if (node && fac) {  // level 0: the original couplings of node k
-          tload<T, NT, W>(Bt, Rk, n, q, false);
-          tload<T, NT, W>(Dt, Lck, n, q, false);
+        // ---- P1: loads (tiles in pairs, each pair's loads in flight together) ----
+        constexpr int EPL = BShape<NT, W>::EPL;  // tile / product elements per lane
+        {
+          // level 0 (node): B <- R_k, D <- Lc_k (original couplings);
+          // level > 0: B <- R_{k-h}, C <- Lc_{k+h}, D <- Lc_{k-h}
+          const T* srcB = up ? Rl : ((node && fac) ? Rk : nullptr);
+          const T* srcD = up ? (fac ? Lcl : nullptr) : ((node && fac) ? Lck : nullptr);
+          if constexpr (EPL <= 4) {  // pairs of tiles in flight
+            T va[EPL], vb[EPL];
+            tfetch<T, NT, W>(va, Lk, n, q);
+            tfetch<T, NT, W>(vb, srcB, n, q);
+            tcommit<T, NT, W>(At, va, Lk, n, q, true);
+            if (up || (node && fac)) tcommit<T, NT, W>(Bt, vb, srcB, n, q, false);
+            if (up || (node && fac)) {
+              tfetch<T, NT, W>(va, up ? Lcr : nullptr, n, q);
+              tfetch<T, NT, W>(vb, srcD, n, q);
+              if (up) tcommit<T, NT, W>(Ct, va, Lcr, n, q, false);
+              if (fac) tcommit<T, NT, W>(Dt, vb, srcD, n, q, false);
+            }
+          } else {  // 16 elements per lane: one tile at a time (registers)
+            tload<T, NT, W>(At, Lk, n, q, true);
+            if (up || (node && fac)) tload<T, NT, W>(Bt, srcB, n, q, false);
+            if (up) tload<T, NT, W>(Ct, Lcr, n, q, false);
+            if (fac && (up || node)) tload<T, NT, W>(Dt, srcD, n, q, false);
+          }
         }
+        // level > 0: the old couplings of node k are read at the fill
+        // products' output positions (the fill-in region is zero on entry; it
+        // is read for parity with the accumulating reference update), each
+        // just before the product that needs it
+        T old[EPL];
         if (fwd) {
           yload<T, NT, W>(Y0, yp, yk, n, m, q);
           if (up) {
@@ -285,9 +370,6 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
         __syncthreads();
 
         // ---- P2: updates from level l-1 ----
-        constexpr int EPL = BShape<NT, W>::EPL;  // product outputs per lane
-        T keep[EPL];
-        int ki[EPL], kj[EPL];
         int nk = 0;
         if (up) {
           if (fac)
@@ -300,22 +382,34 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
                 n, m, n, [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Y1[kk * yp + j]; },
                 right, n, [&](int i, int kk) { return Ct[kk * TP + i]; },
                 [&](int kk, int j) { return Y2[kk * yp + j]; }, [&](int i, int j, T v) { Y0[i * yp + j] -= v; });
-          if (fac && has_lc)
+          if (fac && has_lc) {
+            pfetch<T, NT, W>(old, Lck, n);
+            pcommit<T, NT, W>(old, Lck, n);
             tprod1<T, NT, W>(
                 [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Dt[kk * TP + j]; },
                 [&](int i, int j, T v) {
-                  keep[nk] = (i < n && j < n) ? Lck[(size_t)i * n + j] - v : T(0);
-                  ki[nk] = i;
-                  kj[nk] = j;
+                  old[nk] = (i < n && j < n) ? old[nk] - v : T(0);
                   ++nk;
                 });
+          }
         }
         __syncthreads();
 
         // ---- P3: finish the fill-in of Lc_k, stage R_{k+h}; deferred stores ----
         if (up && fac && has_lc)
-          for (int e = 0; e < nk; ++e) Dt[ki[e] * TP + kj[e]] = keep[e];
-        if (up && fac) tload<T, NT, W>(Bt, Rr, n, q, false);
+#pragma unroll
+          for (int e = 0; e < EPL; ++e) {
+            int i, j;
+            tprod_pos<T, NT, W>(e, i, j);
+            Dt[i * TP + j] = old[e];
+          }
+        if (up && fac) {  // stage R_{k+h}; fetch the old R_k with it
+          T vb[EPL];
+          tfetch<T, NT, W>(vb, Rr, n, q);
+          if (has_r) pfetch<T, NT, W>(old, Rk, n);
+          tcommit<T, NT, W>(Bt, vb, Rr, n, q, false);
+          if (has_r) pcommit<T, NT, W>(old, Rk, n);
+        }
         if (defer) {
           if (fac)
             for (int idx = q; idx < n * n; idx += LPT) Lk[idx] = At[(idx / n) * TP + idx % n];
@@ -331,13 +425,17 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
             tprod1<T, NT, W>(
                 [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Ct[kk * TP + j]; },
                 [&](int i, int j, T v) {
-                  keep[nk] = (i < n && j < n) ? Rk[(size_t)i * n + j] - v : T(0);
-                  ki[nk] = i;
-                  kj[nk] = j;
+                  old[nk] = (i < n && j < n) ? old[nk] - v : T(0);
                   ++nk;
                 });
           __syncthreads();
-          for (int e = 0; e < nk; ++e) Bt[ki[e] * TP + kj[e]] = keep[e];
+          if (has_r)
+#pragma unroll
+            for (int e = 0; e < EPL; ++e) {
+              int i, j;
+              tprod_pos<T, NT, W>(e, i, j);
+              Bt[i * TP + j] = old[e];
+            }
           __syncthreads();
         }
 
@@ -387,9 +485,21 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
         const bool right = node && i + s < N, left = node && i >= s;
         T* Li = node ? L + (size_t)i * nn : nullptr;
         T* xi = node ? x + (size_t)i * nm : nullptr;
-        tload<T, NT, W>(At, Li, n, q, true);
-        tload<T, NT, W>(Bt, right ? E + (size_t)(off + i / s) * nn : nullptr, n, q, false);       // R_i
-        tload<T, NT, W>(Ct, left ? E + (size_t)(off + (i - s) / s) * nn : nullptr, n, q, false);  // Lc_i
+        {
+          const T* Ri = right ? E + (size_t)(off + i / s) * nn : nullptr;       // R_i
+          const T* Lci = left ? E + (size_t)(off + (i - s) / s) * nn : nullptr;  // Lc_i
+          if constexpr (BShape<NT, W>::EPL <= 4) {
+            T va[BShape<NT, W>::EPL], vb[BShape<NT, W>::EPL];
+            tfetch<T, NT, W>(va, Li, n, q);
+            tfetch<T, NT, W>(vb, Ri, n, q);
+            tcommit<T, NT, W>(At, va, Li, n, q, true);
+            tcommit<T, NT, W>(Bt, vb, Ri, n, q, false);
+          } else {
+            tload<T, NT, W>(At, Li, n, q, true);
+            tload<T, NT, W>(Bt, Ri, n, q, false);
+          }
+          tload<T, NT, W>(Ct, Lci, n, q, false);
+        }
         yload<T, NT, W>(Y0, yp, xi, n, m, q);
         yload<T, NT, W>(Y1, yp, right ? x + (size_t)(i + s) * nm : nullptr, n, m, q);
         yload<T, NT, W>(Y2, yp, left ? x + (size_t)(i - s) * nm : nullptr, n, m, q);

@@ -197,6 +197,9 @@ __global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, 
   // Issue every load before storing any: upper tiles of U (diagonal entries
   // turned into reciprocals, lower parts of diagonal tiles zeroed) and the
   // 32-wide chunk of R or Lc.
+  // (All loads are unconditional from clamped addresses and pinned only
+  // after the last one is issued, so they are in flight together; the
+  // masking and reciprocals follow.)
   constexpr int PT = 32 * 32 / kThreads;
   T uv[NT][PT];
 #pragma unroll
@@ -207,8 +210,7 @@ __global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, 
     for (int e = 0; e < PT; ++e) {
       const int idx = tid + e * kThreads;
       const int R = 32 * kb + (idx >> 5), C = 32 * q + (idx & 31);
-      const T g = A[(size_t)min(R, n - 1) * n + min(C, n - 1)];
-      uv[t][e] = (R < n && C < n) ? ((R < C) ? g : ((R == C) ? T(1) / g : T(0))) : ((R == C) ? T(1) : T(0));
+      uv[t][e] = A[(size_t)min(R, n - 1) * n + min(C, n - 1)];
     }
   }
   constexpr int PZ = 32 * NP / kThreads;
@@ -216,15 +218,37 @@ __global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, 
 #pragma unroll
   for (int e = 0; e < PZ; ++e) {
     const int idx = tid + e * kThreads;
-    if (isR) {
-      const int r = idx / NP, k = idx - r * NP;  // R[c0+r][k], row reads
-      const T g = X[(size_t)(c0 + min(r, w - 1)) * n + min(k, n - 1)];
-      zv[e] = (r < w && k < n) ? g : T(0);
-    } else {
-      const int k = idx >> 5, c = idx & 31;  // Lc[k][c0+c]
-      const T g = X[(size_t)min(k, n - 1) * n + c0 + min(c, w - 1)];
-      zv[e] = (c < w && k < n) ? g : T(0);
+    const int rr = idx / NP, kr = idx - rr * NP;  // R[c0+rr][kr], row reads
+    const int kc = idx >> 5, cc = idx & 31;       // Lc[kc][c0+cc]
+    const size_t ar = (size_t)(c0 + min(rr, w - 1)) * n + min(kr, n - 1);
+    const size_t ac = (size_t)min(kc, n - 1) * n + c0 + min(cc, w - 1);
+    zv[e] = X[isR ? ar : ac];
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int e = 0; e < PT; ++e) pin_reg(uv[t][e]);
+#pragma unroll
+  for (int e = 0; e < PZ; ++e) pin_reg(zv[e]);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    int q = 0, kb = t;
+    while (kb > q) { kb -= q + 1; ++q; }
+#pragma unroll
+    for (int e = 0; e < PT; ++e) {
+      const int idx = tid + e * kThreads;
+      const int R = 32 * kb + (idx >> 5), C = 32 * q + (idx & 31);
+      const T g = uv[t][e];
+      uv[t][e] = (R < n && C < n) ? ((R < C) ? g : ((R == C) ? T(1) / g : T(0))) : ((R == C) ? T(1) : T(0));
     }
+  }
+#pragma unroll
+  for (int e = 0; e < PZ; ++e) {
+    const int idx = tid + e * kThreads;
+    const int rr = idx / NP, kr = idx - rr * NP;
+    const int kc = idx >> 5, cc = idx & 31;
+    const bool ok = isR ? (rr < w && kr < n) : (cc < w && kc < n);
+    zv[e] = ok ? zv[e] : T(0);
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t)
@@ -547,14 +571,22 @@ __global__ __launch_bounds__(kThreads) void k_apply_fwd(const T* __restrict__ L,
   const int tid = threadIdx.x;
   const int r = tid % NP, part = tid / NP;
   const bool act = part < PARTS;
+  // All loads are unconditional (clamped addresses) and issued before any is
+  // consumed: pinning them to registers only after the last one is issued
+  // keeps the compiler from turning "cond ? load : 0" into one branch (and
+  // one memory round trip) per load.  Masking follows.
   T a[KP];
+#pragma unroll
+  for (int e = 0; e < KP; ++e) a[e] = U[(size_t)min(part * KP + e, n - 1) * n + min(r, n - 1)];
+  T dg = U[(size_t)min(r, n - 1) * n + min(r, n - 1)];
+#pragma unroll
+  for (int e = 0; e < KP; ++e) pin_reg(a[e]);
+  pin_reg(dg);
 #pragma unroll
   for (int e = 0; e < KP; ++e) {
     const int k = part * KP + e;
-    const T g = U[(size_t)min(k, n - 1) * n + min(r, n - 1)];
-    a[e] = (act && k < r && r < n) ? g : T(0);
+    a[e] = (act && k < r && r < n) ? a[e] : T(0);
   }
-  const T dg = U[(size_t)min(r, n - 1) * n + min(r, n - 1)];
   for (int q = tid; q < NP * w; q += kThreads) {
     const int k = q / w, c = q - k * w;
     Y[k * kRP + c] = (k < n) ? y[(size_t)k * m + c0 + c] : T(0);
@@ -605,13 +637,20 @@ __global__ __launch_bounds__(kThreads) void k_apply_bwd(const T* __restrict__ L,
   const int sub = tid & 7, rr = tid >> 3;
   T a[PASSES][KPL];
 #pragma unroll
+  for (int p = 0; p < PASSES; ++p)
+#pragma unroll
+    for (int e = 0; e < KPL; ++e) a[p][e] = U[(size_t)min(rr + 32 * p, n - 1) * n + min(sub * KPL + e, n - 1)];
+#pragma unroll
+  for (int p = 0; p < PASSES; ++p)
+#pragma unroll
+    for (int e = 0; e < KPL; ++e) pin_reg(a[p][e]);  // all loads in flight before any use
+#pragma unroll
   for (int p = 0; p < PASSES; ++p) {
     const int r = rr + 32 * p;
 #pragma unroll
     for (int e = 0; e < KPL; ++e) {
       const int k = sub * KPL + e;
-      const T g = U[(size_t)min(r, n - 1) * n + min(k, n - 1)];
-      a[p][e] = (r < n && k < n && k > r) ? g : T(0);
+      a[p][e] = (r < n && k < n && k > r) ? a[p][e] : T(0);
     }
   }
   for (int q = tid; q < NP * w; q += kThreads) {

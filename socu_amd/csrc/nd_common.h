@@ -35,6 +35,12 @@ struct Level {
   int next_off;  // first E index of level l+1's segment
 };
 
+// Ties a loaded value to a register at this point of the program, so the
+// compiler keeps the load unconditional and issued with its neighbours
+// instead of sinking it into a later conditional use.
+__device__ __forceinline__ void pin_reg(float& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void pin_reg(double& v) { asm volatile("" : "+v"(v)); }
+
 __host__ __device__ inline int recursive_iterations(int N) {
   int L = 0;
   while (N > 0) { ++L; N >>= 1; }

@@ -166,13 +166,32 @@ __global__ __launch_bounds__(kThreads) void k_residual(const double* __restrict_
 #pragma unroll
       for (int c = 0; c < kResM; ++c) acc[c] = 0.0;
       if (r < n) {
-#pragma unroll 4
-        for (int k = cg; k < n; k += 8) {
-          const double lv = (k > r) ? Li[(size_t)k * n + r] : 0.0;
-          const double ev = Ep ? Ep[(size_t)k * n + r] : 0.0;
+        // chunks of 4 inner indices: unconditional loads (clamped), pinned
+        // after the chunk is issued, masked afterwards
+        const double* Epr = Ep ? Ep : Li;
+        for (int kb = cg; kb < n; kb += 32) {
+          double lv[4], ev[4];
 #pragma unroll
-          for (int c = 0; c < kResM; ++c)
-            if (c < w) acc[c] += lv * x_cur[(size_t)k * m + c0 + c] + ev * x_next[(size_t)k * m + c0 + c];
+          for (int u = 0; u < 4; ++u) {
+            const int k = min(kb + 8 * u, n - 1);
+            lv[u] = Li[(size_t)k * n + r];
+            ev[u] = Epr[(size_t)k * n + r];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            pin_reg(lv[u]);
+            pin_reg(ev[u]);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int k = kb + 8 * u;
+            if (k < n) {
+              const double l1 = (k > r) ? lv[u] : 0.0, e1 = Ep ? ev[u] : 0.0;
+#pragma unroll
+              for (int c = 0; c < kResM; ++c)
+                if (c < w) acc[c] += l1 * x_cur[(size_t)k * m + c0 + c] + e1 * x_next[(size_t)k * m + c0 + c];
+            }
+          }
         }
       }
 #pragma unroll
@@ -184,13 +203,30 @@ __global__ __launch_bounds__(kThreads) void k_residual(const double* __restrict_
     {
       const int r = r0 + rr;
       if (r < n) {
-#pragma unroll 4
-        for (int k = rs; k < n; k += 8) {
-          const double lv = (k <= r) ? Li[(size_t)r * n + k] : 0.0;
-          const double ev = Em ? Em[(size_t)r * n + k] : 0.0;
+        const double* Emr = Em ? Em : Li;
+        for (int kb = rs; kb < n; kb += 32) {
+          double lv[4], ev[4];
 #pragma unroll
-          for (int c = 0; c < kResM; ++c)
-            if (c < w) racc[c] += lv * x_cur[(size_t)k * m + c0 + c] + ev * x_prev[(size_t)k * m + c0 + c];
+          for (int u = 0; u < 4; ++u) {
+            const int k = min(kb + 8 * u, n - 1);
+            lv[u] = Li[(size_t)r * n + k];
+            ev[u] = Emr[(size_t)r * n + k];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            pin_reg(lv[u]);
+            pin_reg(ev[u]);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int k = kb + 8 * u;
+            if (k < n) {
+              const double l1 = (k <= r) ? lv[u] : 0.0, e1 = Em ? ev[u] : 0.0;
+#pragma unroll
+              for (int c = 0; c < kResM; ++c)
+                if (c < w) racc[c] += l1 * x_cur[(size_t)k * m + c0 + c] + e1 * x_prev[(size_t)k * m + c0 + c];
+            }
+          }
         }
       }
 #pragma unroll
