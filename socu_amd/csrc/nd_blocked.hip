@@ -74,9 +74,53 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
   }
   __syncthreads();
 
+  // Off-diagonal tiles of the inverse:
+  //   Linv_ij = -Dinv_i S_ij,  S_ij = sum_{k=j}^{i-1} L_ik Linv_kj  (j < i),
+  //   Linv_jj = Dinv_j.
+  // Linv_ij is kept transposed in the upper tile (j, i):
+  // S[32j+c][32i+r] = Linv_ij[r][c]; S_ij is parked there first and then
+  // overwritten in place.  Column j belongs to wavefront 1 + j, which forms
+  // its blocks while wavefront 0 runs the pivot chain of the next diagonal
+  // tile: in the step of tile kb it finishes Linv_{kb-1,j} and accumulates
+  // S_{kb,j}, whose operands (L_{kb,k} for k < kb, Linv_kj for k < kb) are
+  // final by then.
+  auto inv_partial = [&](int i, int j) {  // S_ij -> upper tile (j, i)
+    typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+    for (int k = j; k < i; ++k) {
+      const T* Lik = S + 32 * i * P + 32 * k;
+      if (k == j) {
+        const T* Dj = S + 32 * j * P + 32 * j;
+        wave_gemm_f<T>(
+            acc, [&](int r, int kk) { return Lik[r * P + kk]; },
+            [&](int kk, int c) { return (kk > c) ? Dj[c * P + kk] : ((kk == c) ? dinv[32 * j + c] : T(0)); }, 32);
+      } else {
+        const T* Ukj = S + 32 * j * P + 32 * k;  // holds Linv_kj transposed
+        wave_gemm<T>(acc, Lik, P, 1, Ukj, 1, P, 32);
+      }
+    }
+    T* Uji = S + 32 * j * P + 32 * i;
+    acc_foreach<T>(acc, [&](int r, int c, T v) { Uji[c * P + r] = v; });
+  };
+  auto inv_finish = [&](int i, int j) {  // Linv_ij = -Dinv_i S_ij, in place
+    T* Uji = S + 32 * j * P + 32 * i;
+    typename Mfma<T>::acc_t inv[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+    const T* Di = S + 32 * i * P + 32 * i;
+    wave_gemm_f<T>(
+        inv, [&](int r, int kk) { return (r > kk) ? Di[kk * P + r] : ((r == kk) ? dinv[32 * i + r] : T(0)); },
+        [&](int kk, int c) { return Uji[c * P + kk]; }, 32);
+    acc_foreach<T>(inv, [&](int r, int c, T v) { Uji[c * P + r] = -v; });
+  };
+
   for (int kb = 0; kb < NB; ++kb) {
     T* D = S + 32 * kb * P + 32 * kb;
-    if (wave == 0) tile_chol_inv<T>(D, P, dinv + 32 * kb, col);
+    if (wave == 0) {
+      tile_chol_inv<T>(D, P, dinv + 32 * kb, col);
+    } else {
+      for (int j = wave - 1; j < NB - 1; j += kThreads / 64 - 1) {
+        if (j < kb - 1) inv_finish(kb - 1, j);
+        if (j <= kb - 1) inv_partial(kb, j);
+      }
+    }
     __syncthreads();
     // panel: tile (ti, kb) <- tile (ti, kb) Linv^T
     for (int ti = kb + 1 + wave; ti < NB; ti += kThreads / 64) {
@@ -115,39 +159,10 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
     __syncthreads();
   }
 
-  // Off-diagonal tiles of the inverse, column by column:
-  //   Linv_ij = -Dinv_i sum_{k=j}^{i-1} L_ik Linv_kj   (j < i),  Linv_jj = Dinv_j.
-  // Wave w owns columns j = w, w+4, ...; Linv_ij is kept transposed in the
-  // upper tile (j, i): S[32j+c][32i+r] = Linv_ij[r][c].  The partial sum is
-  // parked there too, then overwritten in place by the same wave.
-  for (int j = wave; j < NB - 1; j += kThreads / 64) {
-    for (int i = j + 1; i < NB; ++i) {
-      typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
-      for (int k = j; k < i; ++k) {
-        const T* Lik = S + 32 * i * P + 32 * k;
-        if (k == j) {
-          const T* Dj = S + 32 * j * P + 32 * j;
-          wave_gemm_f<T>(
-              acc, [&](int r, int kk) { return Lik[r * P + kk]; },
-              [&](int kk, int c) {
-                return (kk > c) ? Dj[c * P + kk] : ((kk == c) ? dinv[32 * j + c] : T(0));
-              },
-              32);
-        } else {
-          const T* Ukj = S + 32 * j * P + 32 * k;  // holds Linv_kj transposed
-          wave_gemm<T>(acc, Lik, P, 1, Ukj, 1, P, 32);
-        }
-      }
-      T* Uji = S + 32 * j * P + 32 * i;
-      acc_foreach<T>(acc, [&](int r, int c, T v) { Uji[c * P + r] = v; });
-      typename Mfma<T>::acc_t inv[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
-      const T* Di = S + 32 * i * P + 32 * i;
-      wave_gemm_f<T>(
-          inv, [&](int r, int kk) { return (r > kk) ? Di[kk * P + r] : ((r == kk) ? dinv[32 * i + r] : T(0)); },
-          [&](int kk, int c) { return Uji[c * P + kk]; }, 32);
-      acc_foreach<T>(inv, [&](int r, int c, T v) { Uji[c * P + r] = -v; });
-    }
-  }
+  // The last row of inverse blocks (the others were formed during the
+  // pivot chains above).
+  if (wave > 0)
+    for (int j = wave - 1; j < NB - 1; j += kThreads / 64 - 1) inv_finish(NB - 1, j);
   __syncthreads();
 
   // Whole block written back as one contiguous run: tril(L) is the factor;
