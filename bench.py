@@ -27,6 +27,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes moved per factor by the potrf kernel of the mixed path at
+# n=128, N=512 (FP32 storage): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
+# scaled by the same counters on a plain copy of known size; per-level
+# numbers and the commands in profiles/potrf_traffic_n128_N512_pmc.md.
+MEASURED_TRAFFIC = {("potrf", 128, 512, 1): 70036 * 1024}
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
@@ -326,7 +331,8 @@ def main():
         "peak": peak,
         "unit": unit,
         "frac": round(achieved / peak, 5),
-        "traffic": None,
+        "traffic": MEASURED_TRAFFIC.get((dom, n, N, m)),
+        "traffic_algorithmic_bytes": N * 2 * n * n * 4 if dom == "potrf" else None,
         "algorithmic_per_step": per,
         "launches_per_step": kernel_launch[dom],
         "ms_per_step": round(kernel_ms[dom], 4),
