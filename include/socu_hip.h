@@ -87,6 +87,22 @@ int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const u
 int socu_plan_execute(socu_plan* plan, void* stream);
 int socu_plan_destroy(socu_plan* plan);
 
+/* Substitution sweeps with one right-hand side (m == 1): a plan (and a mixed
+ * handle) may own a small workspace and run the upper ND levels of forward
+ * and backward substitution as one persistent launch of node tasks that wait
+ * on per-node counters; the wide lower levels keep the level kernels (same
+ * arithmetic per block as the reference's sweeps,
+ * block_tridiag_solver.py:425-485 and :581-643).  Chosen where it measured
+ * faster (FP32 blocks with n > 96); the environment variable SOCU_SWEEP read
+ * at creation overrides: "level" (never) or "persistent" (always).  Each wait
+ * is bounded; if one expires the launch drains with undefined results and
+ * records it.  socu_plan_sweep_status synchronises `stream` and reports
+ * whether the plan uses the persistent sweeps (*persistent) and whether a
+ * wait expired in its last execution (*expired).  Executions of one plan must
+ * not overlap (they share the workspace).  (No counterpart in the
+ * reference.) */
+int socu_plan_sweep_status(socu_plan* plan, int* persistent, int* expired, void* stream);
+
 /* Mixed-precision factor_and_solve with FP64 iterative refinement (no
  * counterpart in the reference; an extension with the same data layout).
  * L, E (FP64, the matrix Psi) are left untouched; x holds b on entry and the
@@ -108,6 +124,8 @@ int socu_mixed_create(socu_mixed** out, const void* L, const void* E, void* x, c
 int socu_mixed_execute(socu_mixed* handle, void* stream);
 int socu_mixed_stats(socu_mixed* handle, double* rel_resid, int* solves, void* stream);
 int socu_mixed_destroy(socu_mixed* handle);
+/* As socu_plan_sweep_status, for the FP32 correction solves of a handle. */
+int socu_mixed_sweep_status(socu_mixed* handle, int* persistent, int* expired, void* stream);
 /* Runs the mixed solve once without the graph, with the per-kernel-class
  * event timer of socu_profile_op (classes 0-4 as there, 5 = FP64 residual and
  * refinement updates), and synchronises `stream`. */

@@ -57,12 +57,14 @@ SIGNATURES = {
     "socu_plan_create": (_i, [ctypes.POINTER(_vp), _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i]),
     "socu_plan_execute": (_i, [_vp, _vp]),
     "socu_plan_destroy": (_i, [_vp]),
+    "socu_plan_sweep_status": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _vp]),
     "socu_profile_op": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp,
                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
     "socu_mixed_create": (_i, [ctypes.POINTER(_vp), _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, ctypes.c_double, _i, _i]),
     "socu_mixed_execute": (_i, [_vp, _vp]),
     "socu_mixed_stats": (_i, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), _vp]),
     "socu_mixed_destroy": (_i, [_vp]),
+    "socu_mixed_sweep_status": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _vp]),
     "socu_mixed_profile": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
     "socu_batched_max_block_size": (_i, []),
     "socu_batched_create": (_i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(BatchSystem), _i, _vp, _i, _i, _i]),

@@ -211,6 +211,18 @@ class _Launch:
             rc = self._lib.socu_plan_execute(self._plan, ctypes.c_void_p(st.cuda_stream))
         _lib.check(rc, self._name)
 
+    def sweep_status(self):
+        """(persistent, expired): whether the substitution sweeps run as one
+        persistent launch (one right-hand side), and whether a bounded
+        dependency wait expired in the last call.  Synchronises the stream."""
+        st = self._stream if self._stream is not None else torch.cuda.current_stream(self._device)
+        p, e = ctypes.c_int(), ctypes.c_int()
+        with torch.cuda.device(self._device):
+            rc = self._lib.socu_plan_sweep_status(self._plan, ctypes.byref(p), ctypes.byref(e),
+                                                  ctypes.c_void_p(st.cuda_stream))
+        _lib.check(rc, self._name)
+        return bool(p.value), bool(e.value)
+
     def __del__(self):
         plan = getattr(self, "_plan", None)
         if plan is not None and plan.value:

@@ -85,8 +85,30 @@ int run_backward(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N
   return SOCU_OK;
 }
 
+// Workspace of the persistent sweeps (owned by a plan); both NULL selects
+// the level-synchronous sweep kernels.
+struct SweepWs {
+  int* ws = nullptr;
+  void* slots = nullptr;
+};
+
 template <typename T>
-int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N, int n, int m, hipStream_t st) {
+int run_sweeps(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m, bool fwd, bool bwd,
+               const SweepWs& sw, hipStream_t st) {
+  if (sw.ws != nullptr && m == 1) {
+    hipError_t e = socu::sweep_persistent<T>(L, E, x, mask, B, N, n, fwd, bwd, sw.ws, static_cast<T*>(sw.slots), st);
+    if (e != hipSuccess) return hip_fail(e, "persistent sweep launch");
+    return SOCU_OK;
+  }
+  int rc = SOCU_OK;
+  if (fwd) rc = run_forward<T>(L, E, x, mask, B, N, n, m, st);
+  if (rc == SOCU_OK && bwd) rc = run_backward<T>(L, E, x, mask, B, N, n, m, st);
+  return rc;
+}
+
+template <typename T>
+int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N, int n, int m, const SweepWs& sw,
+           hipStream_t st) {
   T* Lt = static_cast<T*>(L);
   T* Et = static_cast<T*>(E);
   T* xt = static_cast<T*>(x);
@@ -95,19 +117,15 @@ int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N,
     case SOCU_OP_FACTOR:
       return run_factor<T>(Lt, Et, mask, B, N, n, st);
     case SOCU_OP_FORWARD:
-      return run_forward<T>(Lt, Et, xt, mask, B, N, n, m, st);
+      return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, true, false, sw, st);
     case SOCU_OP_BACKWARD:
-      return run_backward<T>(Lt, Et, xt, mask, B, N, n, m, st);
+      return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, false, true, sw, st);
     case SOCU_OP_SOLVE:
-      rc = run_forward<T>(Lt, Et, xt, mask, B, N, n, m, st);
-      if (rc) return rc;
-      return run_backward<T>(Lt, Et, xt, mask, B, N, n, m, st);
+      return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, true, true, sw, st);
     case SOCU_OP_FACTOR_AND_SOLVE:
       rc = run_factor<T>(Lt, Et, mask, B, N, n, st);
       if (rc) return rc;
-      rc = run_forward<T>(Lt, Et, xt, mask, B, N, n, m, st);
-      if (rc) return rc;
-      return run_backward<T>(Lt, Et, xt, mask, B, N, n, m, st);
+      return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, true, true, sw, st);
     default:
       return fail(SOCU_EINVAL, "unknown operation %d", op);
   }
@@ -120,13 +138,14 @@ int ensure_init() {
   hipError_t e = socu::init_blocked_kernels();
   if (e == hipSuccess) e = socu::init_refine_kernels();
   if (e == hipSuccess) e = socu::init_batched_kernels();
+  if (e == hipSuccess) e = socu::init_sweep_kernels();
   if (e != hipSuccess) return hip_fail(e, "kernel set-up (hipFuncSetAttribute)");
   state = 1;
   return SOCU_OK;
 }
 
 int dispatch(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m, int dtype,
-             void* stream) {
+             void* stream, const SweepWs& sw = SweepWs()) {
   const bool need_x = op != SOCU_OP_FACTOR;
   int rc = check_dims(L, E, x, need_x, B, N, n, m, dtype);
   if (rc) return rc;
@@ -134,8 +153,8 @@ int dispatch(int op, void* L, void* E, void* x, const unsigned char* mask, int B
   rc = ensure_init();
   if (rc) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == SOCU_DTYPE_F64) return run_op<double>(op, L, E, x, mask, B, N, n, m, st);
-  return run_op<float>(op, L, E, x, mask, B, N, n, m, st);
+  if (dtype == SOCU_DTYPE_F64) return run_op<double>(op, L, E, x, mask, B, N, n, m, sw, st);
+  return run_op<float>(op, L, E, x, mask, B, N, n, m, sw, st);
 }
 
 // Per-thread launch timer used by socu_profile_op.
@@ -178,7 +197,46 @@ struct socu_plan {
   int B, N, n, m, dtype;
   hipGraph_t graph;
   hipGraphExec_t exec;
+  SweepWs sweep;  // persistent-sweep workspace (m == 1 solves), else NULL
 };
+
+namespace {
+
+void free_sweep_ws(SweepWs& sw) {
+  if (sw.ws) (void)hipFree(sw.ws);
+  if (sw.slots) (void)hipFree(sw.slots);
+  sw = SweepWs();
+}
+
+// Allocates the persistent-sweep workspace of a plan whose operation
+// includes a substitution with one right-hand side.
+int alloc_sweep_ws(SweepWs& sw, int op, int B, int N, int n, int m, int dtype) {
+  const size_t esz = dtype == SOCU_DTYPE_F64 ? sizeof(double) : sizeof(float);
+  if (op == SOCU_OP_FACTOR || m != 1 || B <= 0 || !socu::sweep_persistent_enabled(n, (int)esz)) return SOCU_OK;
+  hipError_t e = hipMalloc(&sw.ws, sizeof(int) * socu::sweep_ws_ints(B, N));
+  if (e == hipSuccess) e = hipMemset(sw.ws, 0, sizeof(int) * socu::sweep_ws_ints(B, N));  // status reads 0 before a run
+  if (e == hipSuccess) e = hipMalloc(&sw.slots, esz * socu::sweep_slot_elems(B, N, n));
+  if (e != hipSuccess) {
+    free_sweep_ws(sw);
+    return hip_fail(e, "hipMalloc (sweep workspace)");
+  }
+  return SOCU_OK;
+}
+
+int sweep_status(int* ws, int* persistent, int* expired, void* stream) {
+  if (persistent == nullptr || expired == nullptr) return fail(SOCU_EINVAL, "output pointers are NULL");
+  *persistent = ws != nullptr ? 1 : 0;
+  *expired = 0;
+  if (ws == nullptr) return SOCU_OK;
+  hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+  int st = 0;
+  if (e == hipSuccess) e = hipMemcpy(&st, ws + 1, sizeof(int), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "reading the sweep status");
+  *expired = st != 0 ? 1 : 0;
+  return SOCU_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -271,26 +329,34 @@ int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const u
     rc = ensure_init();
     if (rc) return rc;
   }
-  socu_plan* p = new socu_plan{op, L, E, x, mask, B, N, n, need_x ? m : 1, dtype, nullptr, nullptr};
+  socu_plan* p = new socu_plan{op, L, E, x, mask, B, N, n, need_x ? m : 1, dtype, nullptr, nullptr, SweepWs()};
+  rc = alloc_sweep_ws(p->sweep, op, B, N, n, p->m, dtype);
+  if (rc) {
+    delete p;
+    return rc;
+  }
   if (use_graph && B > 0) {
     hipStream_t cs;
     hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
     if (e != hipSuccess) {
+      free_sweep_ws(p->sweep);
       delete p;
       return hip_fail(e, "hipStreamCreate (graph capture)");
     }
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) {
       (void)hipStreamDestroy(cs);
+      free_sweep_ws(p->sweep);
       delete p;
       return hip_fail(e, "hipStreamBeginCapture");
     }
-    rc = dispatch(op, L, E, x, mask, B, N, n, p->m, dtype, cs);
+    rc = dispatch(op, L, E, x, mask, B, N, n, p->m, dtype, cs, p->sweep);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
     if (rc != SOCU_OK || e2 != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
+      free_sweep_ws(p->sweep);
       delete p;
       return rc != SOCU_OK ? rc : hip_fail(e2, "hipStreamEndCapture");
     }
@@ -298,6 +364,7 @@ int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const u
     e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
       (void)hipGraphDestroy(g);
+      free_sweep_ws(p->sweep);
       delete p;
       return hip_fail(e, "hipGraphInstantiate");
     }
@@ -316,13 +383,19 @@ int socu_plan_execute(socu_plan* p, void* stream) {
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
     return SOCU_OK;
   }
-  return dispatch(p->op, p->L, p->E, p->x, p->mask, p->B, p->N, p->n, p->m, p->dtype, stream);
+  return dispatch(p->op, p->L, p->E, p->x, p->mask, p->B, p->N, p->n, p->m, p->dtype, stream, p->sweep);
+}
+
+int socu_plan_sweep_status(socu_plan* p, int* persistent, int* expired, void* stream) {
+  if (p == nullptr) return fail(SOCU_EINVAL, "plan is NULL");
+  return sweep_status(p->sweep.ws, persistent, expired, stream);
 }
 
 int socu_plan_destroy(socu_plan* p) {
   if (p == nullptr) return SOCU_OK;
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
+  free_sweep_ws(p->sweep);
   delete p;
   return SOCU_OK;
 }
@@ -448,6 +521,11 @@ int socu_mixed_profile(socu_mixed* h, void* stream, double* kernel_ms, int* kern
   if (e != hipSuccess) return hip_fail(e, "mixed-precision solve launch (profile)");
   if (e2 != hipSuccess) return hip_fail(e2, "hipStreamSynchronize (profile)");
   return SOCU_OK;
+}
+
+int socu_mixed_sweep_status(socu_mixed* h, int* persistent, int* expired, void* stream) {
+  if (h == nullptr) return fail(SOCU_EINVAL, "handle is NULL");
+  return sweep_status(h->p.sweep_ws, persistent, expired, stream);
 }
 
 int socu_mixed_destroy(socu_mixed* h) {

@@ -807,7 +807,7 @@ hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, i
 
 template <typename T, int NB, int NF>
 static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
-                                  int step, hipStream_t st) {
+                                  int step, hipStream_t st, bool apply_next) {
   const int nE = (int)off_diag_storage_len(N);
   const int mch = (m + kRhsChunk - 1) / kRhsChunk;
   if (B <= 0 || mch <= 0) return hipSuccess;
@@ -820,7 +820,7 @@ static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* m
                          x, mask, N, n, m, nE, lv.s, lv.off);
   }
   const Level nx = level_info(N, step + 1);
-  if (nx.dim > 0)
+  if (apply_next && nx.dim > 0)
     hipLaunchKernelGGL((k_apply_fwd<T, NB, NF>), dim3(nx.dim, mch, B), dim3(kThreads), 0, st, L, x, mask, N, n, m,
                        nx.s);
   kt_end(kKcForward, st);
@@ -829,19 +829,20 @@ static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* m
 
 template <typename T>
 hipError_t forward_step_blocked(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
-                                int step, hipStream_t st) {
+                                int step, hipStream_t st, bool apply_next) {
+  const bool a = apply_next;
   switch (n) {
-    case 32: return forward_step_nb<T, 1, 32>(L, E, x, mask, B, N, n, m, step, st);
-    case 64: return forward_step_nb<T, 2, 64>(L, E, x, mask, B, N, n, m, step, st);
-    case 96: return forward_step_nb<T, 3, 96>(L, E, x, mask, B, N, n, m, step, st);
-    case 128: return forward_step_nb<T, 4, 128>(L, E, x, mask, B, N, n, m, step, st);
+    case 32: return forward_step_nb<T, 1, 32>(L, E, x, mask, B, N, n, m, step, st, a);
+    case 64: return forward_step_nb<T, 2, 64>(L, E, x, mask, B, N, n, m, step, st, a);
+    case 96: return forward_step_nb<T, 3, 96>(L, E, x, mask, B, N, n, m, step, st, a);
+    case 128: return forward_step_nb<T, 4, 128>(L, E, x, mask, B, N, n, m, step, st, a);
     default: break;
   }
   switch ((n + 31) / 32) {
-    case 1: return forward_step_nb<T, 1, 0>(L, E, x, mask, B, N, n, m, step, st);
-    case 2: return forward_step_nb<T, 2, 0>(L, E, x, mask, B, N, n, m, step, st);
-    case 3: return forward_step_nb<T, 3, 0>(L, E, x, mask, B, N, n, m, step, st);
-    case 4: return forward_step_nb<T, 4, 0>(L, E, x, mask, B, N, n, m, step, st);
+    case 1: return forward_step_nb<T, 1, 0>(L, E, x, mask, B, N, n, m, step, st, a);
+    case 2: return forward_step_nb<T, 2, 0>(L, E, x, mask, B, N, n, m, step, st, a);
+    case 3: return forward_step_nb<T, 3, 0>(L, E, x, mask, B, N, n, m, step, st, a);
+    case 4: return forward_step_nb<T, 4, 0>(L, E, x, mask, B, N, n, m, step, st, a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -888,9 +889,9 @@ template hipError_t factor_level_blocked<float>(float*, float*, const uint8_t*, 
 template hipError_t factor_level_blocked<double>(double*, double*, const uint8_t*, int, int, int, int, bool,
                                                  hipStream_t);
 template hipError_t forward_step_blocked<float>(const float*, const float*, float*, const uint8_t*, int, int, int, int,
-                                                int, hipStream_t);
+                                                int, hipStream_t, bool);
 template hipError_t forward_step_blocked<double>(const double*, const double*, double*, const uint8_t*, int, int, int,
-                                                 int, int, hipStream_t);
+                                                 int, int, hipStream_t, bool);
 template hipError_t backward_level_blocked<float>(const float*, const float*, float*, const uint8_t*, int, int, int,
                                                   int, int, hipStream_t);
 template hipError_t backward_level_blocked<double>(const double*, const double*, double*, const uint8_t*, int, int,

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "nd_common.h"
+
 namespace socu {
 
 // Kernel classes for the optional per-launch event timer (socu_profile_op).
@@ -35,15 +37,42 @@ hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, i
                                 hipStream_t st);
 
 // Forward-substitution step: step -1 applies level 0; step l >= 0 applies
-// level l's updates to every receiving block, then level l+1's inverses.
+// level l's updates to every receiving block, then (unless apply_next is
+// false) level l+1's inverses.
 template <typename T>
 hipError_t forward_step_blocked(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
-                                int step, hipStream_t st);
+                                int step, hipStream_t st, bool apply_next = true);
 
 // Backward-substitution kernels of one ND level.
 template <typename T>
 hipError_t backward_level_blocked(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
                                   int lidx, hipStream_t st);
+
+// Persistent dataflow sweeps for one right-hand side (nd_sweep.hip).
+// Workspace: sweep_ws_ints(B, N) ints (ticket, status word, per-node
+// counters; cleared by the launch itself) and sweep_slot_elems(B, N, n)
+// elements of the solve's type (one n-vector per contribution).
+constexpr int kSweepHeader = 4;             // ticket, status, 2 spare
+constexpr int kSweepSpinLimit = 1 << 21;    // polls per dependency wait
+// Contribution slots of one system: two per receiving block and step,
+// sum over levels l of 2 * (N >> (l+1)).
+__host__ __device__ inline int sweep_slots_per_system(int N) {
+  int t = 0;
+  for (int l = 0; (N >> l) > 0; ++l) t += 2 * (N >> (l + 1));
+  return t;
+}
+hipError_t init_sweep_kernels();
+size_t sweep_ws_ints(int B, int N);
+size_t sweep_slot_elems(int B, int N, int n);
+// Forward and/or backward substitution of x (B, N, n, 1) with the factor in
+// L, E; one clear launch and one persistent launch.
+template <typename T>
+hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, bool fwd, bool bwd,
+                            int* ws, T* slots, hipStream_t st);
+// Whether a plan for block size n and element size dsize uses the persistent
+// sweeps: where they measured faster (FP32, n > 96) unless the environment
+// variable SOCU_SWEEP is "level" (never) or "persistent" (always).
+bool sweep_persistent_enabled(int n, int dsize);
 
 // Mixed-precision factor_and_solve with FP64 refinement (nd_refine.hip).
 struct MixedPlan {
@@ -64,6 +93,8 @@ struct MixedPlan {
   double* partial = nullptr;
   uint8_t* active = nullptr;
   int nblk_init = 1;
+  int* sweep_ws = nullptr;     // persistent sweeps (m == 1), else NULL
+  float* sweep_slots = nullptr;
 };
 hipError_t init_refine_kernels();
 

@@ -315,16 +315,23 @@ hipError_t mixed_alloc(MixedPlan& p) {
   const int np = nres > p.nblk_init ? nres : p.nblk_init;
   if ((e = hipMalloc(&p.partial, sizeof(double) * p.B * (size_t)np)) != hipSuccess) return e;
   if ((e = hipMalloc(&p.active, (size_t)p.B)) != hipSuccess) return e;
+  if (p.m == 1 && sweep_persistent_enabled(p.n, (int)sizeof(float))) {
+    if ((e = hipMalloc(&p.sweep_ws, sizeof(int) * sweep_ws_ints(p.B, p.N))) != hipSuccess) return e;
+    if ((e = hipMemset(p.sweep_ws, 0, sizeof(int) * sweep_ws_ints(p.B, p.N))) != hipSuccess) return e;
+    if ((e = hipMalloc(&p.sweep_slots, sizeof(float) * sweep_slot_elems(p.B, p.N, p.n))) != hipSuccess) return e;
+  }
   return hipSuccess;
 }
 
 void mixed_free(MixedPlan& p) {
-  void* ptrs[] = {p.L32, p.E32, p.d32, p.b64, p.state, p.partial, p.active};
+  void* ptrs[] = {p.L32, p.E32, p.d32, p.b64, p.state, p.partial, p.active, p.sweep_ws, p.sweep_slots};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   p.L32 = p.E32 = p.d32 = nullptr;
   p.b64 = p.state = p.partial = nullptr;
   p.active = nullptr;
+  p.sweep_ws = nullptr;
+  p.sweep_slots = nullptr;
 }
 
 hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
@@ -354,12 +361,19 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
                      p.mask);
   const size_t rlds = mixed_residual_lds(n, m);
   for (int it = 0; it < p.max_iter; ++it) {
-    for (int step = -1; step < Lv - 1; ++step)
-      if ((e = forward_step_blocked<float>(p.L32, p.E32, p.d32, p.active, B, N, n, m, step, st)) != hipSuccess)
+    if (p.sweep_ws != nullptr) {
+      // both sweeps in one persistent launch; converged systems are skipped
+      if ((e = sweep_persistent<float>(p.L32, p.E32, p.d32, p.active, B, N, n, true, true, p.sweep_ws, p.sweep_slots,
+                                       st)) != hipSuccess)
         return e;
-    for (int l = Lv - 1; l >= 0; --l)
-      if ((e = backward_level_blocked<float>(p.L32, p.E32, p.d32, p.active, B, N, n, m, l, st)) != hipSuccess)
-        return e;
+    } else {
+      for (int step = -1; step < Lv - 1; ++step)
+        if ((e = forward_step_blocked<float>(p.L32, p.E32, p.d32, p.active, B, N, n, m, step, st)) != hipSuccess)
+          return e;
+      for (int l = Lv - 1; l >= 0; --l)
+        if ((e = backward_level_blocked<float>(p.L32, p.E32, p.d32, p.active, B, N, n, m, l, st)) != hipSuccess)
+          return e;
+    }
     kt_begin(kKcRefine, st);
     hipLaunchKernelGGL(k_refine_update, dim3(refine_blocks(xcount), B), dim3(kThreads), 0, st, p.x, p.d32, xcount,
                        p.state);

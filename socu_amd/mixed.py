@@ -60,6 +60,16 @@ class _MixedLaunch:
         _lib.check(rc, "cholesky_mixed_factor_and_solve stats")
         return np.array(rr[: self._B]), np.array(it[: self._B])
 
+    def sweep_status(self):
+        """(persistent, expired) of the FP32 correction solves, as for the
+        regular launches.  Synchronises the stream."""
+        p, e = ctypes.c_int(), ctypes.c_int()
+        with torch.cuda.device(self._device):
+            rc = self._lib.socu_mixed_sweep_status(self._h, ctypes.byref(p), ctypes.byref(e),
+                                                   ctypes.c_void_p(self._st().cuda_stream))
+        _lib.check(rc, "mixed sweep status")
+        return bool(p.value), bool(e.value)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

@@ -1,0 +1,251 @@
+"""Persistent dataflow sweeps (one right-hand side) against the oracle.
+
+Plans with m == 1 can run the upper ND levels of forward and backward
+substitution as one persistent launch of node tasks (by default only for FP32
+blocks with n > 96; SOCU_SWEEP=persistent forces it, which every test here
+does unless it compares with the level kernels); this file checks that path block for block against
+oracle.nd (forward result y and solution x; FP64 to 1e-11 relative, FP32 to
+2e-4), against the level-by-level sweep kernels (selected with
+SOCU_SWEEP=level), under masks, in graph replays, for run-to-run
+reproducibility, and through the mixed-precision solve.  Every launch also
+reports that the persistent path ran and that no bounded wait expired.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import nd
+
+pytestmark = pytest.mark.gpu
+
+# (n, N, B): horizons with 1 to 10 levels, block sizes in each of the four
+# register layouts (n <= 32, 64, 96, 128) on and off the 32 multiples.
+CASES = [(1, 1, 1), (1, 2, 2), (5, 3, 1), (8, 7, 3), (16, 20, 2), (32, 33, 1), (33, 17, 2), (64, 64, 1),
+         (96, 9, 2), (100, 31, 1), (128, 40, 2), (7, 255, 1), (3, 1000, 1)]
+
+
+def _rel(a, b):
+    d = np.linalg.norm((a - b).ravel())
+    s = max(np.linalg.norm(b.ravel()), 1e-300)
+    return d / s
+
+
+def _problem(n, N, B, seed):
+    # alpha = 16 (the benchmark's value): with alpha = 10 the generator's
+    # matrices lose definiteness for n >~ 100 (the oracle's own factor then
+    # breaks down), which would test conditioning, not the sweeps.
+    Ls, Es, bs = [], [], []
+    for k in range(B):
+        L0, E0, b0, _, _ = nd.readme_problem(n, N, 1, alpha=16.0, seed=seed + k)
+        Ls.append(L0)
+        Es.append(E0)
+        bs.append(b0)
+    return np.stack(Ls), np.stack(Es), np.stack(bs)
+
+
+class _SweepMode:
+    """Selects the sweep implementation of the plans created inside."""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        self.old = os.environ.get("SOCU_SWEEP")
+        if self.mode is None:
+            os.environ.pop("SOCU_SWEEP", None)
+        else:
+            os.environ["SOCU_SWEEP"] = self.mode
+
+    def __exit__(self, *exc):
+        if self.old is None:
+            os.environ.pop("SOCU_SWEEP", None)
+        else:
+            os.environ["SOCU_SWEEP"] = self.old
+
+
+@pytest.fixture(autouse=True)
+def _force_persistent():
+    with _SweepMode("persistent"):
+        yield
+
+
+def _persistent_ok(launch):
+    persistent, expired = launch.sweep_status()
+    assert persistent and not expired
+
+
+@pytest.mark.parametrize("n,N,B", CASES)
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_forward_backward_solve_match_oracle(n, N, B, dtype, device):
+    import torch
+    from socu_amd.block_tridiag_solver import (
+        create_cholesky_backward_substitution_launch,
+        create_cholesky_factor_launch,
+        create_cholesky_forward_substitution_launch,
+        create_cholesky_solve_launch,
+    )
+
+    tol = 1e-11 if dtype == "float64" else 2e-4
+    tdt = getattr(torch, dtype)
+    L0, E0, b0 = _problem(n, N, B, seed=300 + n + N)
+    Lr, Er, yr = L0.copy(), E0.copy(), b0.copy()
+    nd.factor(Lr, Er)
+    nd.forward(Lr, Er, yr)
+    xr = yr.copy()
+    nd.backward(Lr, Er, xr)
+
+    L = torch.tensor(L0, dtype=tdt, device=device)
+    E = torch.tensor(E0, dtype=tdt, device=device)
+    create_cholesky_factor_launch(L, E, dtype=tdt)()
+    x = torch.tensor(b0, dtype=tdt, device=device)
+    fwd = create_cholesky_forward_substitution_launch(L, E, x, dtype=tdt)
+    fwd()
+    _persistent_ok(fwd)
+    assert _rel(x.cpu().numpy().astype(np.float64), yr) < tol * 10
+    bwd = create_cholesky_backward_substitution_launch(L, E, x, dtype=tdt)
+    bwd()
+    _persistent_ok(bwd)
+    assert _rel(x.cpu().numpy().astype(np.float64), xr) < tol * 100
+
+    x2 = torch.tensor(b0, dtype=tdt, device=device)
+    sol = create_cholesky_solve_launch(L, E, x2, dtype=tdt)
+    sol()
+    _persistent_ok(sol)
+    assert _rel(x2.cpu().numpy().astype(np.float64), xr) < tol * 100
+
+    # the level-by-level kernels on the same factor agree
+    x3 = torch.tensor(b0, dtype=tdt, device=device)
+    with _SweepMode("level"):
+        lev = create_cholesky_solve_launch(L, E, x3, dtype=tdt)
+    assert lev.sweep_status() == (False, False)
+    lev()
+    torch.cuda.synchronize()
+    assert _rel(x2.cpu().numpy().astype(np.float64), x3.cpu().numpy().astype(np.float64)) < tol * 10
+
+
+def test_bench_size_residual(device):
+    """n=128, N=512 (the benchmark shape): factor_and_solve in FP64 through
+    the persistent sweeps reaches the same residual as the level kernels."""
+    import torch
+    from socu_amd import problems
+    from socu_amd.block_tridiag_solver import create_cholesky_factor_and_solve_launch
+
+    L0, E0, b0 = problems.readme_problem(128, 512, 1, alpha=16.0, seed=0)
+    out = {}
+    for mode in ("persistent", "level"):
+        L = torch.tensor(L0, device=device)
+        E = torch.tensor(E0, device=device)
+        x = torch.tensor(b0, device=device)
+        with _SweepMode(mode):
+            run = create_cholesky_factor_and_solve_launch(L, E, x)
+        run()
+        torch.cuda.synchronize()
+        out[mode] = x.cpu().numpy()
+        if mode == "persistent":
+            _persistent_ok(run)
+    r0 = problems.rel_residual(L0, E0, out["persistent"], b0)
+    r1 = problems.rel_residual(L0, E0, out["level"], b0)
+    assert r0 < 1e-12 and r1 < 1e-12, (r0, r1)
+    assert _rel(out["persistent"], out["level"]) < 1e-11
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_mask_leaves_entries_untouched(dtype, device):
+    import torch
+    from socu_amd.block_tridiag_solver import create_cholesky_factor_launch, create_cholesky_solve_launch
+
+    tdt = getattr(torch, dtype)
+    n, N, B = 24, 37, 4
+    L0, E0, b0 = _problem(n, N, B, seed=51)
+    L = torch.tensor(L0, dtype=tdt, device=device)
+    E = torch.tensor(E0, dtype=tdt, device=device)
+    create_cholesky_factor_launch(L, E, dtype=tdt)()
+    mask = torch.tensor([True, False, True, False], device=device)
+    x = torch.tensor(b0, dtype=tdt, device=device)
+    run = create_cholesky_solve_launch(L, E, x, mask=mask, dtype=tdt)
+    run()
+    _persistent_ok(run)
+    Lr, Er = L0.copy(), E0.copy()
+    nd.factor(Lr, Er)
+    xg = x.cpu().numpy().astype(np.float64)
+    tol = 1e-9 if dtype == "float64" else 2e-2
+    for k in range(B):
+        if k % 2:
+            assert torch.equal(x[k].cpu(), torch.tensor(b0[k], dtype=tdt))
+        else:
+            xr = b0[k].copy()
+            nd.solve(Lr[k], Er[k], xr)
+            assert _rel(xg[k], xr) < tol
+
+
+def test_graph_replays_and_reproducibility(device):
+    """A captured solve replayed three times (inputs restored in between)
+    equals the eager result bit for bit: contributions are summed in a fixed
+    order, never with atomics."""
+    import torch
+    from socu_amd.block_tridiag_solver import create_cholesky_factor_launch, create_cholesky_solve_launch
+
+    n, N, B = 40, 150, 2
+    L0, E0, b0 = _problem(n, N, B, seed=77)
+    L = torch.tensor(L0, device=device)
+    E = torch.tensor(E0, device=device)
+    create_cholesky_factor_launch(L, E)()
+    xe = torch.tensor(b0, device=device)
+    eager = create_cholesky_solve_launch(L, E, xe)
+    eager()
+    torch.cuda.synchronize()
+    ref = xe.clone()
+    for _ in range(2):
+        xe.copy_(torch.tensor(b0, device=device))
+        eager()
+        torch.cuda.synchronize()
+        assert torch.equal(xe, ref)
+    xg = torch.tensor(b0, device=device)
+    graph = create_cholesky_solve_launch(L, E, xg, use_cuda_graph=True)
+    for rep in range(3):
+        xg.copy_(torch.tensor(b0, device=device))
+        torch.cuda.synchronize()
+        graph()
+        torch.cuda.synchronize()
+        assert torch.equal(xg, ref), rep
+    _persistent_ok(graph)
+
+
+def test_default_selection(device):
+    """Without an override the plans pick the persistent sweeps exactly for
+    FP32 blocks with n > 96 and one right-hand side."""
+    import torch
+    from socu_amd.block_tridiag_solver import create_cholesky_solve_launch
+
+    with _SweepMode(None):
+        for n, dt, m, want in [(128, torch.float32, 1, True), (97, torch.float32, 1, True),
+                               (96, torch.float32, 1, False), (128, torch.float64, 1, False),
+                               (128, torch.float32, 2, False)]:
+            L = torch.zeros(1, 3, n, n, dtype=dt, device=device)
+            E = torch.zeros(1, 2, n, n, dtype=dt, device=device)
+            x = torch.zeros(1, 3, n, m, dtype=dt, device=device)
+            run = create_cholesky_solve_launch(L, E, x, dtype=dt)
+            assert run.sweep_status() == (want, False), (n, dt, m)
+
+
+def test_mixed_solve_uses_persistent_sweeps(device):
+    import torch
+    from socu_amd import problems
+    from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
+
+    L0, E0, b0 = problems.readme_problem(64, 200, 1, alpha=16.0, seed=3)
+    res = {}
+    for mode in ("persistent", "level"):
+        L = torch.tensor(L0, device=device)
+        E = torch.tensor(E0, device=device)
+        x = torch.tensor(b0, device=device)
+        with _SweepMode(mode):
+            run = create_cholesky_mixed_factor_and_solve_launch(L, E, x, tol=1e-12, max_iter=5, use_cuda_graph=True)
+        run()
+        torch.cuda.synchronize()
+        assert run.sweep_status() == ((mode == "persistent"), False)
+        rr, solves = run.stats()
+        res[mode] = (x.cpu().numpy(), rr[0], solves[0])
+    assert res["persistent"][1] <= 1e-12 and res["level"][1] <= 1e-12
+    assert problems.rel_residual(L0, E0, res["persistent"][0], b0) <= 1e-12
