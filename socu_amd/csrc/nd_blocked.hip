@@ -693,7 +693,7 @@ __global__ __launch_bounds__(kThreads) void k_apply_bwd(const T* __restrict__ L,
 // ---------------------------------------------------------------------------
 template <typename T, int NB>
 constexpr size_t potrf_blk_lds() {
-  return sizeof(T) * ((size_t)(32 * NB) * (32 * NB + 1) + 32 * NB + 32);
+  return sizeof(T) * ((size_t)(32 * NB) * (32 * NB + 1) + 32 * NB + 128);
 }
 template <typename T, int NB>
 constexpr size_t trsm_blk_lds() {

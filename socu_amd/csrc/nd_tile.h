@@ -136,14 +136,21 @@ __device__ __forceinline__ void acc16_foreach(typename Mfma<T>::acc_t& acc, F f)
 // S: tile in LDS, row-major with `pitch`; its lower triangle is the input.
 // On return the lower triangle holds L, the strict upper triangle holds the
 // transposed inverse (S[c][q] = Linv[q][c] for q > c) and dinv[r] = 1/L[r][r].
+// col: 128 entries of wave-private LDS (two buffers of two columns).
 //
 // Lanes 0-31 keep row r of the tile in u[]; lanes 32-63 keep column r of the
-// inverse (Gauss-Jordan: Linv = L_31^-1 ... L_0^-1).  Step k publishes column
-// k through the wave-private LDS buffer `col` (LDS operations of one
-// wavefront complete in issue order, so no barrier is needed); the pivot
-// comes from lane k by readlane, so the serial chain per step is one
-// readlane and one reciprocal square root.  Both halves then apply the same
-// update u[q] -= coef * col[q] with their own coefficient.
+// inverse (Gauss-Jordan: Linv = L_31^-1 ... L_0^-1).  The pivots are taken two
+// at a time: for the 2x2 pivot block [a b; b c] (from lanes k, k+1 by
+// readlane), 1/sqrt(a) and 1/sqrt(ac - b^2) are computed side by side and
+// 1/sqrt(c - b^2/a) = sqrt(a) / sqrt(ac - b^2) follows with two products, so
+// the serial chain per two columns is one reciprocal square root.  Steps k
+// and k+1 then fuse into one rank-2 update per entry,
+//   u[q] -= (c1 - c2 t) A[q][k] + c2 A[q][k+1],   t = b / a,
+// with each half's coefficients c1 (step k) and c2 (step k+1, from the entry
+// k+1 updated by step k).  Columns k, k+1 come from the LDS buffer (k/2) % 2
+// (LDS operations of one wavefront complete in issue order, so no barrier is
+// needed).  Look-ahead: entries k+2, k+3 are updated first and published as
+// the next pair's columns before the remaining updates.
 template <typename T>
 __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) {
   const int lane = lane_id();
@@ -156,18 +163,65 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
     const T v = S[r * pitch + c];  // in-bounds; upper entries are discarded
     u[c] = rowlane ? ((c <= r) ? v : T(0)) : ((c == r) ? T(1) : T(0));
   }
+  // one base address in a vector register: every column access below is
+  // that base plus an immediate offset
+  typedef __attribute__((address_space(3))) T lds_t;
+  lds_t* colv = (lds_t*)col;
+  asm volatile("" : "+v"(colv));
+  if (rowlane) {
+    colv[r] = u[0];
+    colv[32 + r] = u[1];
+  }
 #pragma unroll
-  for (int k = 0; k < 32; ++k) {
-    if (rowlane) col[r] = u[k];
-    const T p = bcast_lane(u[k], k);
-    const T inv = rsqrt_t(p);
-    // row lanes: l = L[r][k] (0 for r <= k); inverse lanes: z_k = Linv[k][r]
-    const T lk = rowlane ? ((r > k) ? u[k] * inv : T(0)) : u[k] * inv;
-    const T coef = lk * inv;  // L[q][k] = col[q] * inv
+  for (int k = 0; k < 32; k += 2) {
+    const lds_t* cb = colv + 64 * ((k >> 1) & 1);
+    lds_t* nb = colv + 64 * (((k >> 1) + 1) & 1);
+    // columns k and k+1, read while the pivots' reciprocal square roots are
+    // computed
+    T c0[32], c1[32];
 #pragma unroll
-    for (int q = k + 1; q < 32; ++q) u[q] -= coef * col[q];
-    u[k] = rowlane ? ((r == k) ? p * inv : ((r > k) ? lk : u[k])) : lk;
-    rinv = (r == k) ? inv : rinv;
+    for (int q = k + 2; q < 32; ++q) {
+      c0[q] = cb[q];
+      c1[q] = cb[32 + q];
+    }
+    const T a = bcast_lane(u[k], k);
+    const T b = bcast_lane(u[k], k + 1);
+    const T c = bcast_lane(u[k + 1], k + 1);
+    const T inv1 = rsqrt_t(a);
+    const T g = rsqrt_t(a * c - b * b);
+    const T inv2 = g * (a * inv1);  // 1 / sqrt(c - b^2 / a)
+    const T t = b * inv1 * inv1;
+    // step k: row lanes l = L[r][k] (0 for r <= k); inverse lanes z_k
+    const T keep1 = (rowlane && r <= k) ? T(0) : T(1);
+    const T lk1 = u[k] * inv1 * keep1;
+    const T cf1 = lk1 * inv1;
+    // step k+1, on entry k+1 as updated by step k (A[k+1][k] = b)
+    const T uk1 = u[k + 1] - cf1 * b;
+    const T keep2 = (rowlane && r <= k + 1) ? T(0) : T(1);
+    const T lk2 = uk1 * inv2 * keep2;
+    const T cf2 = lk2 * inv2;
+    const T cf1p = cf1 - cf2 * t;
+    // (explicit fused multiply-adds: two per entry and pair of steps)
+    T ncf1 = -cf1p, ncf2 = -cf2;
+    if (k + 2 < 32) {
+      u[k + 2] = fma(ncf2, c1[k + 2], fma(ncf1, c0[k + 2], u[k + 2]));
+      u[k + 3] = fma(ncf2, c1[k + 3], fma(ncf1, c0[k + 3], u[k + 3]));
+      asm volatile("" : "+v"(u[k + 2]), "+v"(u[k + 3]));
+      if (rowlane) {
+        nb[r] = u[k + 2];
+        nb[32 + r] = u[k + 3];
+      }
+      // the remaining updates use the coefficients as pinned here, after the
+      // stores
+      asm volatile("" : "+v"(ncf1), "+v"(ncf2) : : "memory");
+    }
+#pragma unroll
+    for (int q = k + 4; q < 32; ++q) u[q] = fma(ncf2, c1[q], fma(ncf1, c0[q], u[q]));
+    const T pk1 = a * inv1;                // sqrt(a)
+    const T pk2 = (c - b * t) * inv2;      // sqrt(c - b^2 / a)
+    u[k] = rowlane ? ((r == k) ? pk1 : ((r > k) ? lk1 : u[k])) : lk1;
+    u[k + 1] = rowlane ? ((r == k + 1) ? pk2 : ((r > k + 1) ? lk2 : uk1)) : lk2;
+    rinv = (r == k) ? inv1 : ((r == k + 1) ? inv2 : rinv);
     // Materialise the registers every step: otherwise the optimiser sinks
     // the updates to their last use and keeps every broadcast value alive.
 #pragma unroll
