@@ -21,6 +21,8 @@
 //                row-split block-times-panel updates, then triangular GEMVs
 //                with the stored inverses (replaces K9-K16,
 //                block_tridiag_solver.py:425-736).
+#include <stdlib.h>
+
 #include "nd_common.h"
 #include "nd_internal.h"
 #include "nd_tile.h"
@@ -450,6 +452,149 @@ __global__ __launch_bounds__(kThreads) void k_schur_blk(T* __restrict__ L, T* __
 }
 
 // ---------------------------------------------------------------------------
+// Schur-complement updates for levels with few nodes.  k_schur_blk walks K in
+// chunks, one global round trip per chunk, which at the upper levels (a
+// handful of tiles) is the whole cost.  Here a task is a 32x32 output tile
+// (4x more tasks) and issues every load it needs at once: the 32-wide panels
+// of its operands (all of K) and its own C values, then one LDS staging and
+// the MFMA products.  Panels are kept as P[i][k] (pitch NP + 1): row panels
+// are 32 rows of a block, column panels 32 of its columns, transposed.
+// ---------------------------------------------------------------------------
+template <typename T, int NB>
+constexpr size_t schur_small_lds() {
+  return sizeof(T) * 4 * 32 * (size_t)(32 * NB + 1);
+}
+
+template <typename T, int NB>
+__device__ __forceinline__ void panel_load(T* v, const T* __restrict__ M, bool cols, int i0, int n) {
+  constexpr int NP = 32 * NB;
+  constexpr int PE = 32 * NP / kThreads;
+#pragma unroll
+  for (int e = 0; e < PE; ++e) {
+    const int idx = threadIdx.x + e * kThreads;
+    int i, k;
+    if (!cols) {  // row panel: contiguous along k
+      i = idx / NP;
+      k = idx - i * NP;
+    } else {      // column panel: 32 contiguous entries of each row k
+      k = idx >> 5;
+      i = idx & 31;
+    }
+    const int rr = cols ? k : i0 + i, cc = cols ? i0 + i : k;
+    v[e] = M[min(rr, n - 1) * n + min(cc, n - 1)];
+  }
+}
+
+template <typename T, int NB>
+__device__ __forceinline__ void panel_store(T* P, const T* v, bool cols, int i0, int n) {
+  constexpr int NP = 32 * NB;
+  constexpr int PE = 32 * NP / kThreads;
+#pragma unroll
+  for (int e = 0; e < PE; ++e) {
+    const int idx = threadIdx.x + e * kThreads;
+    int i, k;
+    if (!cols) {
+      i = idx / NP;
+      k = idx - i * NP;
+    } else {
+      k = idx >> 5;
+      i = idx & 31;
+    }
+    P[i * (NP + 1) + k] = (i0 + i < n && k < n) ? v[e] : T(0);
+  }
+}
+
+template <typename T, int NB, int NF>
+__global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* __restrict__ E,
+                                                         const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
+                                                         int off, int next_off, int nT) {
+  const int n = NF ? NF : n_;
+  constexpr int NP = 32 * NB, PP = NP + 1;
+  constexpr int PE = 32 * NP / kThreads;
+  constexpr int NTRI = NB * (NB + 1) / 2;
+  const int b = blockIdx.y;
+  if (masked_out(mask, b)) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Pn = reinterpret_cast<T*>(smem_raw);  // four panels of 32 x PP
+  const size_t nn = (size_t)n * n;
+  T* Eb = E + (size_t)b * nE * nn;
+  int task = blockIdx.x;
+  const T* M[4];
+  bool cols[4];
+  int i0[4];
+  int np;
+  T* C;
+  int tr, tc;
+  bool diag;
+  if (task < nT * NTRI) {
+    // L_j -= R R^T + Lc^T Lc  (R = block (j, j-s), Lc = block (j+s, j))
+    const int u = task / NTRI;
+    int q = task - u * NTRI;
+    tr = 0;
+    while (q > tr) { q -= tr + 1; ++tr; }
+    tc = q;
+    const int j = 2 * s * (u + 1) - 1;
+    C = L + ((size_t)b * N + j) * nn;
+    const T* R = Eb + (size_t)(off + (j - s) / s) * nn;
+    const bool hasC = j + s < N;
+    const T* Lc = hasC ? Eb + (size_t)(off + j / s) * nn : R;
+    M[0] = R;  M[1] = R;  M[2] = Lc;  M[3] = Lc;
+    cols[0] = false; cols[1] = false; cols[2] = true; cols[3] = true;
+    i0[0] = 32 * tr; i0[1] = 32 * tc; i0[2] = 32 * tr; i0[3] = 32 * tc;
+    np = hasC ? 2 : 1;
+    diag = tr == tc;
+  } else {
+    // fill-in F -= R Lc  (node i: R = block (i+s, i), Lc = block (i, i-s))
+    task -= nT * NTRI;
+    const int f = task / (NB * NB);
+    const int q = task - f * NB * NB;
+    tr = q / NB;
+    tc = q - tr * NB;
+    const int i = s - 1 + 2 * s * (f + 1);
+    const T* R = Eb + (size_t)(off + i / s) * nn;
+    const T* Lc = Eb + (size_t)(off + (i - s) / s) * nn;
+    C = Eb + (size_t)(next_off + (i - s) / (2 * s)) * nn;
+    M[0] = R;  M[1] = Lc;  M[2] = R;  M[3] = Lc;
+    cols[0] = false; cols[1] = true; cols[2] = false; cols[3] = true;
+    i0[0] = 32 * tr; i0[1] = 32 * tc; i0[2] = 32 * tr; i0[3] = 32 * tc;
+    np = 1;
+    diag = false;
+  }
+  const int l = lane_id(), wave = wave_id();
+  const int wi = wave >> 1, wj = wave & 1;
+  // every load of the task at once: panels, then this wave's C values
+  T v[4][PE];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+    if (p < 2 * np) panel_load<T, NB>(v[p], M[p], cols[p], i0[p], n);
+  T cv[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int rr = 32 * tr + 16 * wi + Mfma<T>::row(l, r), cc = 32 * tc + 16 * wj + Mfma<T>::col(l, r);
+    cv[r] = C[min(rr, n - 1) * n + min(cc, n - 1)];
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int e = 0; e < PE; ++e) pin_reg(v[p][e]);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) pin_reg(cv[r]);
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+    if (p < 2 * np) panel_store<T, NB>(Pn + p * 32 * PP, v[p], cols[p], i0[p], n);
+  __syncthreads();
+  typename Mfma<T>::acc_t acc = acc_zero<T>();
+  for (int p = 0; p < np; ++p)
+    wave_gemm16<T>(acc, Pn + (2 * p) * 32 * PP + 16 * wi * PP, PP, 1, Pn + (2 * p + 1) * 32 * PP + 16 * wj * PP, 1,
+                   PP, NP);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int rr = 32 * tr + 16 * wi + Mfma<T>::row(l, r), cc = 32 * tc + 16 * wj + Mfma<T>::col(l, r);
+    if (rr < n && cc < n && (!diag || cc <= rr)) C[rr * n + cc] = cv[r] - acc[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // solves, with the explicit inverses of the diagonal blocks (see k_potrf_blk)
 //
 // Forward step l (levels ascending):
@@ -719,6 +864,32 @@ static hipError_t set_blk_attrs() {
                              (int)trsm_blk_lds<T, NB>());
 }
 
+template <typename T, int NB>
+static hipError_t set_schur_small_attrs() {
+  hipError_t e = hipFuncSetAttribute((const void*)k_schur_small<T, NB, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)schur_small_lds<T, NB>());
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_schur_small<T, NB, 32 * NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)schur_small_lds<T, NB>());
+}
+
+// Whether a level's Schur updates (nodes = targets + fill-ins over the batch)
+// take k_schur_small.  Its panel loads repeat each operand per output tile,
+// which only costs where the level streams many blocks; measured on MI355X
+// (N=512, 1024) the small kernel wins up to about 12 MiB of the level's
+// blocks: n=128 FP64 factor 1207 -> 1093 us, FP32 875 -> 767 us, n=32 FP64
+// 265 -> 197 us.  SOCU_SCHUR_SMALL=<nodes> replaces the rule by a node count
+// (0 disables the small kernel).
+static bool use_schur_small(int nodes, int n, int esize) {
+  static int env = -2;
+  if (env == -2) {
+    const char* e = getenv("SOCU_SCHUR_SMALL");
+    env = e ? (atoi(e) < 0 ? 0 : atoi(e)) : -1;
+  }
+  if (env >= 0) return nodes <= env;
+  return (double)nodes * n * n * esize <= 12.0 * 1024 * 1024;
+}
+
 template <int NB>
 static hipError_t set_mixed_attrs() {
   hipError_t e = hipFuncSetAttribute((const void*)k_potrf_blk<float, NB, 0, double>,
@@ -738,6 +909,14 @@ hipError_t init_blocked_kernels() {
   if ((e = set_blk_attrs<double, 2>()) != hipSuccess) return e;
   if ((e = set_blk_attrs<double, 3>()) != hipSuccess) return e;
   if ((e = set_blk_attrs<double, 4>()) != hipSuccess) return e;
+  if ((e = set_schur_small_attrs<float, 1>()) != hipSuccess) return e;
+  if ((e = set_schur_small_attrs<float, 2>()) != hipSuccess) return e;
+  if ((e = set_schur_small_attrs<float, 3>()) != hipSuccess) return e;
+  if ((e = set_schur_small_attrs<float, 4>()) != hipSuccess) return e;
+  if ((e = set_schur_small_attrs<double, 1>()) != hipSuccess) return e;
+  if ((e = set_schur_small_attrs<double, 2>()) != hipSuccess) return e;
+  if ((e = set_schur_small_attrs<double, 3>()) != hipSuccess) return e;
+  if ((e = set_schur_small_attrs<double, 4>()) != hipSuccess) return e;
   if ((e = set_mixed_attrs<1>()) != hipSuccess) return e;
   if ((e = set_mixed_attrs<2>()) != hipSuccess) return e;
   if ((e = set_mixed_attrs<3>()) != hipSuccess) return e;
@@ -778,8 +957,15 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
   const int tasks = nT * ntri + nF * ntile * ntile;
   if (tasks > 0) {
     kt_begin(kKcSchur, st);
-    hipLaunchKernelGGL((k_schur_blk<T, NF>), dim3(tasks, B), dim3(kThreads), 0, st, L, E, mask, N, n, nE, lv.s, lv.off,
-                       lv.next_off, nT, ntile);
+    if (use_schur_small(B * (nT + nF), n, (int)sizeof(T))) {
+      const int nsmall = nT * (NB * (NB + 1) / 2) + nF * NB * NB;
+      const size_t lds_small = schur_small_lds<T, NB>();
+      hipLaunchKernelGGL((k_schur_small<T, NB, NF>), dim3(nsmall, B), dim3(kThreads), lds_small, st, L, E, mask, N, n,
+                         nE, lv.s, lv.off, lv.next_off, nT);
+    } else {
+      hipLaunchKernelGGL((k_schur_blk<T, NF>), dim3(tasks, B), dim3(kThreads), 0, st, L, E, mask, N, n, nE, lv.s,
+                         lv.off, lv.next_off, nT, ntile);
+    }
     kt_end(kKcSchur, st);
   }
   return hipGetLastError();
