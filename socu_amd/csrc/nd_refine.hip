@@ -21,26 +21,44 @@ namespace socu {
 // elementwise kernels
 // ---------------------------------------------------------------------------
 
-// dst (float) <- src (double) over count elements of each batch entry.
-__global__ __launch_bounds__(kThreads) void k_cast_f64_f32(float* __restrict__ dst, const double* __restrict__ src,
-                                                          size_t count, size_t dst_stride, size_t src_stride,
-                                                          const uint8_t* __restrict__ mask) {
+// FP32 working copy of Psi in one launch: per batch entry
+//   L32 <- L (lc elements), E32[0, ec) <- E (the tridiagonal blocks),
+//   E32[ec, es) <- 0 (fill-in region).
+// VEC: four elements per step (two 16-byte loads, one 16-byte store), for
+// lc, ec, es multiples of 4 and 16-byte aligned arrays.
+template <bool VEC>
+__global__ __launch_bounds__(kThreads) void k_psi_to_f32(float* __restrict__ L32, float* __restrict__ E32,
+                                                        const double* __restrict__ L, const double* __restrict__ E,
+                                                        size_t lc, size_t ec, size_t es,
+                                                        const uint8_t* __restrict__ mask) {
   const int b = blockIdx.y;
   if (masked_out(mask, b)) return;
-  const double* s = src + (size_t)b * src_stride;
-  float* d = dst + (size_t)b * dst_stride;
-  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < count; i += (size_t)gridDim.x * kThreads)
-    d[i] = (float)s[i];
-}
-
-// dst (float) <- 0 over count elements of each batch entry.
-__global__ __launch_bounds__(kThreads) void k_zero_f32(float* __restrict__ dst, size_t count, size_t stride,
-                                                      const uint8_t* __restrict__ mask) {
-  const int b = blockIdx.y;
-  if (masked_out(mask, b)) return;
-  float* d = dst + (size_t)b * stride;
-  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < count; i += (size_t)gridDim.x * kThreads)
-    d[i] = 0.f;
+  constexpr int W = VEC ? 4 : 1;
+  const size_t nl = lc / W, ne = ec / W, total = (lc + es) / W;
+  const double* Lb = L + (size_t)b * lc;
+  const double* Eb = E + (size_t)b * es;
+  float* L32b = L32 + (size_t)b * lc;
+  float* E32b = E32 + (size_t)b * es;
+  for (size_t u = (size_t)blockIdx.x * kThreads + threadIdx.x; u < total; u += (size_t)gridDim.x * kThreads) {
+    if (VEC) {
+      if (u < nl) {
+        const double2 a = reinterpret_cast<const double2*>(Lb)[2 * u];
+        const double2 c = reinterpret_cast<const double2*>(Lb)[2 * u + 1];
+        reinterpret_cast<float4*>(L32b)[u] = make_float4((float)a.x, (float)a.y, (float)c.x, (float)c.y);
+      } else if (u - nl < ne) {
+        const size_t v = u - nl;
+        const double2 a = reinterpret_cast<const double2*>(Eb)[2 * v];
+        const double2 c = reinterpret_cast<const double2*>(Eb)[2 * v + 1];
+        reinterpret_cast<float4*>(E32b)[v] = make_float4((float)a.x, (float)a.y, (float)c.x, (float)c.y);
+      } else {
+        reinterpret_cast<float4*>(E32b)[u - nl] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    } else {
+      if (u < nl) L32b[u] = (float)Lb[u];
+      else if (u - nl < ne) E32b[u - nl] = (float)Eb[u - nl];
+      else E32b[u - nl] = 0.f;
+    }
+  }
 }
 
 // Refinement state per batch entry (4 doubles): [0] = ||b||^2, [1] = ||r||^2
@@ -342,15 +360,23 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
   if (e != hipSuccess) return e;
   // FP32 working copy of Psi; the fill-in region starts at zero.
   const size_t lc = (size_t)N * nn;
-  hipLaunchKernelGGL(k_cast_f64_f32, dim3(refine_blocks(lc), B), dim3(kThreads), 0, st, p.L32, p.L, lc, lc, lc,
-                     p.mask);
-  if (N > 1) {
-    const size_t ec = (size_t)(N - 1) * nn, es = (size_t)nE * nn;
-    hipLaunchKernelGGL(k_cast_f64_f32, dim3(refine_blocks(ec), B), dim3(kThreads), 0, st, p.E32, p.E, ec, es, es,
-                       p.mask);
-    const size_t fc = (size_t)(nE - (N - 1)) * nn;
-    if (fc > 0)
-      hipLaunchKernelGGL(k_zero_f32, dim3(refine_blocks(fc), B), dim3(kThreads), 0, st, p.E32 + ec, fc, es, p.mask);
+  const size_t ec = N > 1 ? (size_t)(N - 1) * nn : 0, es = N > 1 ? (size_t)nE * nn : 0;
+  {
+    const bool vec = lc % 4 == 0 && ec % 4 == 0 && es % 4 == 0 && ((uintptr_t)p.L % 16) == 0 &&
+                     ((uintptr_t)p.E % 16) == 0 && ((uintptr_t)p.L32 % 16) == 0 && ((uintptr_t)p.E32 % 16) == 0;
+    const size_t units = (lc + es) / (vec ? 4 : 1);
+    size_t blocks = (units + kThreads - 1) / kThreads;
+    const size_t cap = (size_t)(2048 / B > 0 ? 2048 / B : 1);
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    const double* Ep = N > 1 ? p.E : p.L;  // unused when es == 0
+    float* E32p = N > 1 ? p.E32 : p.L32;
+    if (vec)
+      hipLaunchKernelGGL((k_psi_to_f32<true>), dim3((unsigned)blocks, B), dim3(kThreads), 0, st, p.L32, E32p, p.L, Ep,
+                         lc, ec, es, p.mask);
+    else
+      hipLaunchKernelGGL((k_psi_to_f32<false>), dim3((unsigned)blocks, B), dim3(kThreads), 0, st, p.L32, E32p, p.L,
+                         Ep, lc, ec, es, p.mask);
   }
   const int Lv = recursive_iterations(N);
   for (int l = 0; l < Lv; ++l)
