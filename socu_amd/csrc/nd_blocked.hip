@@ -113,14 +113,37 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
     acc_foreach<T>(inv, [&](int r, int c, T v) { Uji[c * P + r] = -v; });
   };
 
+  // tile (gi, gj) -= L_{gi,kb} L_{gj,kb}^T
+  auto trail = [&](int kb, int gi, int gj) {
+    typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+    wave_gemm<T>(acc, S + 32 * gi * P + 32 * kb, P, 1, S + 32 * gj * P + 32 * kb, 1, P, 32);
+    T* C = S + 32 * gi * P + 32 * gj;
+    acc_foreach<T>(acc, [&](int r, int c, T v) { C[r * P + c] -= v; });
+  };
+
+  // Two phases per diagonal tile kb, with look-ahead: wave 0 applies the
+  // previous step's update to tile (kb, kb) only, then factors it; meanwhile
+  // waves 1..3 apply the rest of that update (tiles (gi, gj), kb <= gj <= gi,
+  // except (kb, kb)) and the inverse-block work.  Then every wave forms the
+  // panel tiles (ti, kb) <- tile (ti, kb) Linv_kb^T.
   for (int kb = 0; kb < NB; ++kb) {
     T* D = S + 32 * kb * P + 32 * kb;
     if (wave == 0) {
+      if (kb > 0) trail(kb - 1, kb, kb);
       tile_chol_inv<T>(D, P, dinv + 32 * kb, col);
     } else {
       for (int j = wave - 1; j < NB - 1; j += kThreads / 64 - 1) {
         if (j < kb - 1) inv_finish(kb - 1, j);
         if (j <= kb - 1) inv_partial(kb, j);
+      }
+      if (kb > 0) {
+        const int m = NB - kb;
+        const int cnt = m * (m + 1) / 2;  // triangle index 0 is (kb, kb)
+        for (int t = wave; t < cnt; t += kThreads / 64 - 1) {
+          int ti = 0, q = t;
+          while (q > ti) { q -= ti + 1; ++ti; }
+          trail(kb - 1, kb + ti, kb + q);
+        }
       }
     }
     __syncthreads();
@@ -143,20 +166,6 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
         acc[1][1] = Mfma<T>::mma(a1, b1, acc[1][1]);
       }
       acc_foreach<T>(acc, [&](int r, int c, T v) { X[r * P + c] = v; });
-    }
-    __syncthreads();
-    // trailing update of the lower tiles (ti, tj), kb < tj <= ti
-    const int m = NB - 1 - kb;
-    const int cnt = m * (m + 1) / 2;
-    for (int t = wave; t < cnt; t += kThreads / 64) {
-      int ti = 0, q = t;
-      while (q > ti) { q -= ti + 1; ++ti; }
-      const int tj = q;
-      const int gi = kb + 1 + ti, gj = kb + 1 + tj;
-      typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
-      wave_gemm<T>(acc, S + 32 * gi * P + 32 * kb, P, 1, S + 32 * gj * P + 32 * kb, 1, P, 32);
-      T* C = S + 32 * gi * P + 32 * gj;
-      acc_foreach<T>(acc, [&](int r, int c, T v) { C[r * P + c] -= v; });
     }
     __syncthreads();
   }
