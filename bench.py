@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--alpha", type=float, default=16.0)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--max-iter", type=int, default=3)
+    ap.add_argument("--precision", default="float32_f32_pivots", choices=["float32", "float32_f32_pivots"],
+                    help="FP32 factor with FP32 (default) or FP64 diagonal-block pivots; the other one is timed too")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the batched and gradient workloads")
@@ -267,8 +269,8 @@ def main():
 
     # --- mixed FP32 factor + FP64 refinement (the metric) -----------------
     xm = bp.clone()
-    mx = create_cholesky_mixed_factor_and_solve_launch(Lp, Ep, xm, tol=args.tol, max_iter=args.max_iter,
-                                                       use_cuda_graph=True)
+    mx = create_cholesky_mixed_factor_and_solve_launch(Lp, Ep, xm, precision=args.precision, tol=args.tol,
+                                                       max_iter=args.max_iter, use_cuda_graph=True)
     perm, wallm = _time_steps(mx, lambda: xm.copy_(bp), st, args.steps, args.warmup)
     msm = float(perm.mean())
     rr, its = mx.stats()
@@ -279,10 +281,21 @@ def main():
     # the difference is the cost of the converged (no-op) iterations
     xm2 = bp.clone()
     need = max(int(its[0]), 1)
-    mx2 = create_cholesky_mixed_factor_and_solve_launch(Lp, Ep, xm2, tol=args.tol, max_iter=need, use_cuda_graph=True)
+    mx2 = create_cholesky_mixed_factor_and_solve_launch(Lp, Ep, xm2, precision=args.precision, tol=args.tol,
+                                                        max_iter=need, use_cuda_graph=True)
     perm2, _ = _time_steps(mx2, lambda: xm2.copy_(bp), st, args.steps, args.warmup)
     ms_need = float(perm2.mean())
     log(f"bench: mixed+refine with max_iter={need}: {ms_need:.3f} ms")
+    # the other pivot precision, same protocol (reported, not the value)
+    other = "float32" if args.precision == "float32_f32_pivots" else "float32_f32_pivots"
+    xo = bp.clone()
+    mo = create_cholesky_mixed_factor_and_solve_launch(Lp, Ep, xo, precision=other, tol=args.tol,
+                                                       max_iter=args.max_iter, use_cuda_graph=True)
+    pero, _ = _time_steps(mo, lambda: xo.copy_(bp), st, args.steps, args.warmup)
+    rro, itso = mo.stats()
+    other_rec = {"precision": other, "ms": round(float(pero.mean()), 4), "correction_solves": int(itso[0]),
+                 "rel_resid": problems.rel_residual(L0, E0, xo.cpu().numpy(), b0)}
+    log(f"bench: mixed+refine with {other}: {other_rec}")
     km_mixed, kc_mixed = _profile(lib, lambda a, b: _lib.check(lib.socu_mixed_profile(
         mx._h, ctypes.c_void_p(st.cuda_stream), a, b), "mixed profile"), ncls)
     kernel_ms = {k: float(km_mixed[i]) for i, k in enumerate(_lib.KERNEL_CLASSES)}
@@ -321,9 +334,10 @@ def main():
         if dom == "potrf":
             per *= 2  # factor and explicit inverse of each diagonal block
         achieved = per / (kernel_ms[dom] * 1e-3) / 1e12 if kernel_ms[dom] > 0 else 0.0
-        # the mixed potrf computes in FP64 (FP64 pivots); trsm and Schur in FP32
+        # the mixed potrf computes in FP64 with FP64 pivots, else in FP32;
+        # trsm and Schur in FP32
         bound, unit = "mfma", "TFLOP/s"
-        peak = FP64_PEAK_TFLOPS if dom == "potrf" else FP32_PEAK_TFLOPS
+        peak = FP64_PEAK_TFLOPS if (dom == "potrf" and args.precision == "float32") else FP32_PEAK_TFLOPS
     roofline = {
         "kernel": dom,
         "bound": bound,
@@ -363,7 +377,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 factor + f64 refinement",
+        "dtype": "f32 factor" + (" (f32 pivots)" if args.precision == "float32_f32_pivots" else " (f64 pivots)")
+                 + " + f64 refinement",
         "data": "synthetic (README generator, seed 0)",
         "config": {"workload": f"ND block-tridiagonal Cholesky, mixed FP32 factor + FP64 refinement to rel-resid "
                                f"{args.tol:g}, n={n} N={N} m={m} B=1, alpha={args.alpha}",
@@ -372,6 +387,7 @@ def main():
                    "fp64_factor_and_solve_per_s": round(1e3 / ms64, 3), "fp64_ms": round(ms64, 4),
                    "fp64_rel_resid": res64, "speedup_vs_fp64": round(ms64 / msm, 4),
                    "mixed_ms_with_max_iter_needed": round(ms_need, 4), "max_iter": args.max_iter,
+                   "precision": args.precision, "other_precision": other_rec,
                    "fp64_gflops": round((ff + fsv) / (ms64 * 1e-3) / 1e9, 2),
                    "kernel_ms": kernel_ms, "fp64_kernel_ms": {k: round(float(k64[i]), 4)
                                                              for i, k in enumerate(_lib.KERNEL_CLASSES)},

@@ -24,7 +24,8 @@ def _problem(n, N, m, alpha, seed, B=1):
 @pytest.mark.parametrize("n,N,m,alpha", [(32, 20, 1, 10.0), (64, 33, 4, 10.0), (128, 16, 1, 16.0), (36, 7, 3, 10.0),
                                          (128, 64, 8, 16.0), (10, 100, 2, 10.0)])
 @pytest.mark.parametrize("graph", [False, True])
-def test_mixed_reaches_fp64_residual(n, N, m, alpha, graph, device):
+@pytest.mark.parametrize("precision", ["float32", "float32_f32_pivots"])
+def test_mixed_reaches_fp64_residual(n, N, m, alpha, graph, precision, device):
     import torch
     from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
 
@@ -32,7 +33,8 @@ def test_mixed_reaches_fp64_residual(n, N, m, alpha, graph, device):
     L = torch.tensor(L0, device=device)
     E = torch.tensor(E0, device=device)
     x = torch.tensor(b0, device=device)
-    run = create_cholesky_mixed_factor_and_solve_launch(L, E, x, tol=1e-10, max_iter=8, use_cuda_graph=graph)
+    run = create_cholesky_mixed_factor_and_solve_launch(L, E, x, precision=precision, tol=1e-10, max_iter=8,
+                                                        use_cuda_graph=graph)
     for rep in range(2):  # second call: graph replay / re-run on fresh b
         x.copy_(torch.tensor(b0, device=device))
         run()
