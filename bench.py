@@ -28,10 +28,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes moved per factor by the potrf kernel of the mixed path at
-# n=128, N=512 (FP32 storage): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
-# scaled by the same counters on a plain copy of known size; per-level
-# numbers and the commands in profiles/potrf_traffic_n128_N512_pmc.md.
-MEASURED_TRAFFIC = {("potrf", 128, 512, 1): 70036 * 1024}
+# n=128, N=512 (FP32 storage), per pivot precision: rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE, fetch scaled by the write/fetch ratio of the
+# bench's own device copies (write exact); per-launch numbers and the commands
+# in profiles/potrf_traffic_n128_N512_pmc.md.
+MEASURED_TRAFFIC = {("potrf", 128, 512, 1, "float32_f32_pivots"): 69122 * 1024,
+                    ("potrf", 128, 512, 1, "float32"): 69681 * 1024}
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
@@ -320,8 +322,10 @@ def main():
         log(f"bench: literal alpha=10: {literal}")
 
     ff, fsv = problems.nd_flops(N, n, m)
-    # dominant kernel class of the measured (mixed) path; fp32 factor kernels
-    dom = max(kernel_ms, key=kernel_ms.get)
+    # roofline of the largest single kernel of the measured (mixed) path: the
+    # diagonal-block factor (the forward class bundles several kernels: level
+    # updates and applies and the persistent upper-level sweep)
+    dom = "potrf"
     solves = max(int(its[0]), 1)
     if dom in ("forward", "backward", "refine"):
         # sweeps and refinement stream the factor / matrix: HBM-bound
@@ -345,7 +349,7 @@ def main():
         "peak": peak,
         "unit": unit,
         "frac": round(achieved / peak, 5),
-        "traffic": MEASURED_TRAFFIC.get((dom, n, N, m)),
+        "traffic": MEASURED_TRAFFIC.get((dom, n, N, m, args.precision)),
         "traffic_algorithmic_bytes": N * 2 * n * n * 4 if dom == "potrf" else None,
         "algorithmic_per_step": per,
         "launches_per_step": kernel_launch[dom],
