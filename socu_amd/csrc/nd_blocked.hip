@@ -317,6 +317,157 @@ __global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, 
   }
 }
 
+// The same products with a few chunks per workgroup, for wide levels:
+// k_trsm_blk reloads the node's upper tiles of U in each of its 2*NB
+// workgroups (8x the block at n=128), which dominates its HBM traffic when
+// every CU has work.  Here a workgroup stages U once and walks `cpb`
+// consecutive chunks of the node's list (its R row chunks, then its Lc column
+// chunks), the next chunk's loads in flight while the current one is
+// multiplied.
+template <typename T, int NB, int NF>
+__global__ __launch_bounds__(kThreads) void k_trsm_node(const T* __restrict__ L, T* __restrict__ E,
+                                                       const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
+                                                       int off, int cpb) {
+  const int n = NF ? NF : n_;
+  constexpr int NP = 32 * NB;
+  constexpr int TP = 33;
+  constexpr int NT = NB * (NB + 1) / 2;
+  constexpr int XP = NP + 1;
+  constexpr int PT = 32 * 32 / kThreads;
+  constexpr int PZ = 32 * NP / kThreads;
+  const int b = blockIdx.z;
+  if (masked_out(mask, b)) return;
+  const int i = s - 1 + 2 * s * blockIdx.x;
+  const bool hasR = i + s < N, hasC = i >= s;
+  const int nbn = (n + 31) / 32;  // chunks per block
+  const int all = (hasR ? nbn : 0) + (hasC ? nbn : 0);
+  // this workgroup's chunks: [q0, q0 + total) of the node's list
+  const int q0 = blockIdx.y * cpb;
+  const int total = min(cpb, all - q0);
+  if (total <= 0) return;
+  const size_t nn = (size_t)n * n;
+  const T* A = L + ((size_t)b * N + i) * nn;
+  T* XR = hasR ? E + ((size_t)b * nE + off + i / s) * nn : nullptr;        // R_i: block (i+s, i)
+  T* XC = hasC ? E + ((size_t)b * nE + off + (i - s) / s) * nn : nullptr;  // Lc_i: block (i, i-s)
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Ut = reinterpret_cast<T*>(smem_raw);
+  T* Xs = Ut + NT * 32 * TP;
+  const int tid = threadIdx.x, wave = wave_id();
+  // chunk q: R rows 32c.. for q < nbn (if R exists), then Lc columns 32c..
+  auto chunk = [&](int q, bool& isR, int& c0, int& w, T*& X) {
+    isR = hasR && q < nbn;
+    const int c = isR ? q : q - (hasR ? nbn : 0);
+    c0 = 32 * c;
+    w = min(32, n - c0);
+    X = isR ? XR : XC;
+  };
+  auto zload = [&](T* zv, bool isR, int c0, int w, const T* X) {
+#pragma unroll
+    for (int e = 0; e < PZ; ++e) {
+      const int idx = tid + e * kThreads;
+      const int rr = idx / NP, kr = idx - rr * NP;  // R[c0+rr][kr], row reads
+      const int kc = idx >> 5, cc = idx & 31;       // Lc[kc][c0+cc]
+      const size_t ar = (size_t)(c0 + min(rr, w - 1)) * n + min(kr, n - 1);
+      const size_t ac = (size_t)min(kc, n - 1) * n + c0 + min(cc, w - 1);
+      zv[e] = X[isR ? ar : ac];
+    }
+  };
+  auto zstore = [&](T* zv, bool isR, int w) {
+#pragma unroll
+    for (int e = 0; e < PZ; ++e) {
+      const int idx = tid + e * kThreads;
+      const int rr = idx / NP, kr = idx - rr * NP;
+      const int kc = idx >> 5, cc = idx & 31;
+      const bool ok = isR ? (rr < w && kr < n) : (cc < w && kc < n);
+      const T v = ok ? zv[e] : T(0);
+      if (isR) Xs[(idx / NP) * XP + (idx % NP)] = v;
+      else Xs[(idx >> 5) * TP + (idx & 31)] = v;
+    }
+  };
+
+  // U tiles and the first chunk: every load issued before any is used
+  T uv[NT][PT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    int q = 0, kb = t;
+    while (kb > q) { kb -= q + 1; ++q; }
+#pragma unroll
+    for (int e = 0; e < PT; ++e) {
+      const int idx = tid + e * kThreads;
+      const int R = 32 * kb + (idx >> 5), C = 32 * q + (idx & 31);
+      uv[t][e] = A[(size_t)min(R, n - 1) * n + min(C, n - 1)];
+    }
+  }
+  bool isR;
+  int c0, w;
+  T* X;
+  chunk(q0, isR, c0, w, X);
+  T zv[PZ];
+  zload(zv, isR, c0, w, X);
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int e = 0; e < PT; ++e) pin_reg(uv[t][e]);
+#pragma unroll
+  for (int e = 0; e < PZ; ++e) pin_reg(zv[e]);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    int q = 0, kb = t;
+    while (kb > q) { kb -= q + 1; ++q; }
+#pragma unroll
+    for (int e = 0; e < PT; ++e) {
+      const int idx = tid + e * kThreads;
+      const int R = 32 * kb + (idx >> 5), C = 32 * q + (idx & 31);
+      const T g = uv[t][e];
+      Ut[t * 32 * TP + (idx >> 5) * TP + (idx & 31)] =
+          (R < n && C < n) ? ((R < C) ? g : ((R == C) ? T(1) / g : T(0))) : ((R == C) ? T(1) : T(0));
+    }
+  }
+  zstore(zv, isR, w);
+  __syncthreads();
+
+  for (int q = 0; q < total; ++q) {
+    const bool more = q + 1 < total;
+    bool isRn = false;
+    int c0n = 0, wn = 0;
+    T* Xn = nullptr;
+    if (more) {
+      chunk(q0 + q + 1, isRn, c0n, wn, Xn);
+      zload(zv, isRn, c0n, wn, Xn);  // in flight during the products below
+    }
+    for (int qq = wave; qq < NB; qq += kThreads / 64) {
+      typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+      for (int kb = 0; kb <= qq; ++kb) {
+        const T* Utile = Ut + (qq * (qq + 1) / 2 + kb) * 32 * TP;  // U[32kb.., 32qq..]
+        if (isR) wave_gemm<T>(acc, Xs + 32 * kb, XP, 1, Utile, TP, 1, 32);
+        else wave_gemm<T>(acc, Utile, 1, TP, Xs + 32 * kb * TP, TP, 1, 32);
+      }
+      if (isR) {
+        acc_foreach<T>(acc, [&](int r, int c, T v) {
+          const int C = 32 * qq + c;
+          if (r < w && C < n) X[(size_t)(c0 + r) * n + C] = v;
+        });
+      } else {
+        acc_foreach<T>(acc, [&](int r, int c, T v) {
+          const int R = 32 * qq + r;
+          if (c < w && R < n) X[(size_t)R * n + c0 + c] = v;
+        });
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int e = 0; e < PZ; ++e) pin_reg(zv[e]);
+      __syncthreads();  // every wave is done with the current chunk
+      zstore(zv, isRn, wn);
+      __syncthreads();
+      isR = isRn;
+      c0 = c0n;
+      w = wn;
+      X = Xn;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Schur-complement updates, 64x64 output tiles on MFMA
 // ---------------------------------------------------------------------------
@@ -869,8 +1020,39 @@ static hipError_t set_blk_attrs() {
   e = hipFuncSetAttribute((const void*)k_trsm_blk<T, NB, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)trsm_blk_lds<T, NB>());
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)k_trsm_blk<T, NB, 32 * NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  e = hipFuncSetAttribute((const void*)k_trsm_blk<T, NB, 32 * NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)trsm_blk_lds<T, NB>());
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_trsm_node<T, NB, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)trsm_blk_lds<T, NB>());
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_trsm_node<T, NB, 32 * NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)trsm_blk_lds<T, NB>());
+}
+
+// Whether a level's trsm takes k_trsm_node, and with how many chunks per
+// workgroup.  Measured on MI355X (N=512): for n > 96 and levels of at least
+// 64 nodes, two chunks per workgroup cut the n=128 factor by 9 us (FP32) and
+// 35 us (FP64); for n <= 64, or one workgroup per node, it was slower.
+// SOCU_TRSM_NODE=<min nodes> and SOCU_TRSM_CPB=<chunks> override.
+static int trsm_node_min(int nb) {
+  static int v = -2;
+  if (v == -2) {
+    const char* e = getenv("SOCU_TRSM_NODE");
+    v = e ? (atoi(e) < 1 ? 1 : atoi(e)) : -1;
+  }
+  if (v > 0) return v;
+  return nb == 4 ? 64 : (1 << 30);
+}
+
+static int trsm_chunks_per_block() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("SOCU_TRSM_CPB");
+    v = e ? atoi(e) : 2;
+    if (v < 1) v = 1;
+  }
+  return v;
 }
 
 template <typename T, int NB>
@@ -954,8 +1136,15 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
   kt_end(kKcPotrf, st);
   if (lidx == Lv - 1) return hipGetLastError();
   kt_begin(kKcTrsm, st);
-  hipLaunchKernelGGL((k_trsm_blk<T, NB, NF>), dim3(lv.dim, 2 * NB, B), dim3(kThreads), lds_trsm, st, L, E, mask, N, n,
-                     nE, lv.s, lv.off);
+  if (B * lv.dim >= trsm_node_min(NB)) {
+    const int cpb = trsm_chunks_per_block();
+    const int groups = (2 * NB + cpb - 1) / cpb;
+    hipLaunchKernelGGL((k_trsm_node<T, NB, NF>), dim3(lv.dim, groups, B), dim3(kThreads), lds_trsm, st, L, E, mask, N,
+                       n, nE, lv.s, lv.off, cpb);
+  }
+  else
+    hipLaunchKernelGGL((k_trsm_blk<T, NB, NF>), dim3(lv.dim, 2 * NB, B), dim3(kThreads), lds_trsm, st, L, E, mask, N,
+                       n, nE, lv.s, lv.off);
   kt_end(kKcTrsm, st);
   const int ntile = (n + kSTile - 1) / kSTile;
   const int ntri = ntile * (ntile + 1) / 2;

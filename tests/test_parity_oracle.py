@@ -98,3 +98,37 @@ def test_factor_is_deterministic(device):
         outs.append((np.tril(L.cpu().numpy()), E.cpu().numpy(), x.cpu().numpy()))
     for a, b in zip(outs[0], outs[1]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [100, 128])
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_wide_level_factor_matches_oracle(n, dtype, device):
+    """N=130, B=2: the widest levels (130 and 64 nodes over the batch) take the
+    chunked trsm and Schur variants for large blocks; every tril(L) and E block
+    against the oracle (alpha=16: the generator stays well conditioned at
+    n >= 100)."""
+    import torch
+    from socu_amd.block_tridiag_solver import create_cholesky_factor_launch
+
+    tol = 1e-11 if dtype == "float64" else 2e-4
+    tdt = getattr(torch, dtype)
+    N, B = 130, 2
+    Ls, Es = [], []
+    for b in range(B):
+        L0, E0, _, _, _ = nd.readme_problem(n, N, 1, alpha=16.0, seed=500 + b)
+        Ls.append(L0)
+        Es.append(E0)
+    L0, E0 = np.stack(Ls), np.stack(Es)
+    Lr, Er = L0.copy(), E0.copy()
+    nd.factor(Lr, Er)
+    L = torch.tensor(L0, dtype=tdt, device=device)
+    E = torch.tensor(E0, dtype=tdt, device=device)
+    create_cholesky_factor_launch(L, E, dtype=tdt)()
+    Lg = np.tril(L.cpu().numpy().astype(np.float64))
+    Eg = E.cpu().numpy().astype(np.float64)
+    for b in range(B):
+        for i in range(N):
+            assert _rel(Lg[b, i], np.tril(Lr[b, i])) < tol, (b, i)
+        for k in range(Er.shape[1]):
+            if np.linalg.norm(Er[b, k]) > 0:
+                assert _rel(Eg[b, k], Er[b, k]) < tol, (b, k)
