@@ -108,7 +108,7 @@ int run_sweeps(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, 
 
 template <typename T>
 int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N, int n, int m, const SweepWs& sw,
-           hipStream_t st) {
+           const socu::SweepOverlap* ov, hipStream_t st) {
   T* Lt = static_cast<T*>(L);
   T* Et = static_cast<T*>(E);
   T* xt = static_cast<T*>(x);
@@ -123,6 +123,11 @@ int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N,
     case SOCU_OP_SOLVE:
       return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, true, true, sw, st);
     case SOCU_OP_FACTOR_AND_SOLVE:
+      if (ov != nullptr) {  // forward sweep alongside the factor, then the backward sweep
+        const hipError_t e = socu::factor_with_forward<T>(Lt, Et, xt, mask, mask, B, N, n, m, false, *ov, st);
+        if (e != hipSuccess) return hip_fail(e, "factor_and_solve launch");
+        return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, false, true, sw, st);
+      }
       rc = run_factor<T>(Lt, Et, mask, B, N, n, st);
       if (rc) return rc;
       return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, true, true, sw, st);
@@ -145,7 +150,7 @@ int ensure_init() {
 }
 
 int dispatch(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m, int dtype,
-             void* stream, const SweepWs& sw = SweepWs()) {
+             void* stream, const SweepWs& sw = SweepWs(), const socu::SweepOverlap* ov = nullptr) {
   const bool need_x = op != SOCU_OP_FACTOR;
   int rc = check_dims(L, E, x, need_x, B, N, n, m, dtype);
   if (rc) return rc;
@@ -153,8 +158,8 @@ int dispatch(int op, void* L, void* E, void* x, const unsigned char* mask, int B
   rc = ensure_init();
   if (rc) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == SOCU_DTYPE_F64) return run_op<double>(op, L, E, x, mask, B, N, n, m, sw, st);
-  return run_op<float>(op, L, E, x, mask, B, N, n, m, sw, st);
+  if (dtype == SOCU_DTYPE_F64) return run_op<double>(op, L, E, x, mask, B, N, n, m, sw, ov, st);
+  return run_op<float>(op, L, E, x, mask, B, N, n, m, sw, ov, st);
 }
 
 // Per-thread launch timer used by socu_profile_op.
@@ -198,6 +203,8 @@ struct socu_plan {
   hipGraph_t graph;
   hipGraphExec_t exec;
   SweepWs sweep;  // persistent-sweep workspace (m == 1 solves), else NULL
+  socu::SweepOverlap overlap;  // factor_and_solve: forward sweep alongside the factor
+  bool has_overlap;
 };
 
 namespace {
@@ -329,43 +336,45 @@ int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const u
     rc = ensure_init();
     if (rc) return rc;
   }
-  socu_plan* p = new socu_plan{op, L, E, x, mask, B, N, n, need_x ? m : 1, dtype, nullptr, nullptr, SweepWs()};
+  socu_plan* p = new socu_plan{op, L, E, x, mask, B, N, n, need_x ? m : 1, dtype, nullptr, nullptr, SweepWs(),
+                               socu::SweepOverlap(), false};
   rc = alloc_sweep_ws(p->sweep, op, B, N, n, p->m, dtype);
+  if (rc == SOCU_OK && op == SOCU_OP_FACTOR_AND_SOLVE && B > 0) {
+    const hipError_t e = socu::overlap_create(p->overlap);
+    if (e != hipSuccess) rc = hip_fail(e, "stream / event creation (factor_and_solve)");
+    p->has_overlap = e == hipSuccess;
+  }
   if (rc) {
-    delete p;
+    socu_plan_destroy(p);
     return rc;
   }
   if (use_graph && B > 0) {
     hipStream_t cs;
     hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
     if (e != hipSuccess) {
-      free_sweep_ws(p->sweep);
-      delete p;
+      socu_plan_destroy(p);  // workspace and streams; no graph yet
       return hip_fail(e, "hipStreamCreate (graph capture)");
     }
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) {
       (void)hipStreamDestroy(cs);
-      free_sweep_ws(p->sweep);
-      delete p;
+      socu_plan_destroy(p);  // workspace and streams; no graph yet
       return hip_fail(e, "hipStreamBeginCapture");
     }
-    rc = dispatch(op, L, E, x, mask, B, N, n, p->m, dtype, cs, p->sweep);
+    rc = dispatch(op, L, E, x, mask, B, N, n, p->m, dtype, cs, p->sweep, p->has_overlap ? &p->overlap : nullptr);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
     if (rc != SOCU_OK || e2 != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
-      free_sweep_ws(p->sweep);
-      delete p;
+      socu_plan_destroy(p);  // workspace and streams; no graph yet
       return rc != SOCU_OK ? rc : hip_fail(e2, "hipStreamEndCapture");
     }
     hipGraphExec_t ex = nullptr;
     e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
       (void)hipGraphDestroy(g);
-      free_sweep_ws(p->sweep);
-      delete p;
+      socu_plan_destroy(p);  // workspace and streams; no graph yet
       return hip_fail(e, "hipGraphInstantiate");
     }
     p->graph = g;
@@ -383,7 +392,8 @@ int socu_plan_execute(socu_plan* p, void* stream) {
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
     return SOCU_OK;
   }
-  return dispatch(p->op, p->L, p->E, p->x, p->mask, p->B, p->N, p->n, p->m, p->dtype, stream, p->sweep);
+  return dispatch(p->op, p->L, p->E, p->x, p->mask, p->B, p->N, p->n, p->m, p->dtype, stream, p->sweep,
+                  p->has_overlap ? &p->overlap : nullptr);
 }
 
 int socu_plan_sweep_status(socu_plan* p, int* persistent, int* expired, void* stream) {
@@ -396,6 +406,7 @@ int socu_plan_destroy(socu_plan* p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   free_sweep_ws(p->sweep);
+  if (p->has_overlap) socu::overlap_destroy(p->overlap);
   delete p;
   return SOCU_OK;
 }

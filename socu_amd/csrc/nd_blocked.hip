@@ -1117,7 +1117,7 @@ hipError_t init_blocked_kernels() {
 
 template <typename T, int NB, int NF>
 static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, bool f64_pivots,
-                                  hipStream_t st) {
+                                  hipStream_t st, hipEvent_t after_potrf, hipEvent_t after_trsm) {
   const int nE = (int)off_diag_storage_len(N);
   const int Lv = recursive_iterations(N);
   const Level lv = level_info(N, lidx);
@@ -1134,6 +1134,10 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
                        lv.s);
   }
   kt_end(kKcPotrf, st);
+  if (after_potrf != nullptr) {
+    const hipError_t e = hipEventRecord(after_potrf, st);
+    if (e != hipSuccess) return e;
+  }
   if (lidx == Lv - 1) return hipGetLastError();
   kt_begin(kKcTrsm, st);
   if (B * lv.dim >= trsm_node_min(NB)) {
@@ -1146,6 +1150,10 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
     hipLaunchKernelGGL((k_trsm_blk<T, NB, NF>), dim3(lv.dim, 2 * NB, B), dim3(kThreads), lds_trsm, st, L, E, mask, N,
                        n, nE, lv.s, lv.off);
   kt_end(kKcTrsm, st);
+  if (after_trsm != nullptr) {
+    const hipError_t e = hipEventRecord(after_trsm, st);
+    if (e != hipSuccess) return e;
+  }
   const int ntile = (n + kSTile - 1) / kSTile;
   const int ntri = ntile * (ntile + 1) / 2;
   const int nT = N / (2 * lv.s);
@@ -1171,32 +1179,32 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
 
 template <typename T>
 hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, bool f64_pivots,
-                                hipStream_t st) {
+                                hipStream_t st, hipEvent_t after_potrf, hipEvent_t after_trsm) {
   const bool fp = f64_pivots;
   switch (n) {
-    case 32: return factor_level_nb<T, 1, 32>(L, E, mask, B, N, n, lidx, fp, st);
-    case 64: return factor_level_nb<T, 2, 64>(L, E, mask, B, N, n, lidx, fp, st);
-    case 96: return factor_level_nb<T, 3, 96>(L, E, mask, B, N, n, lidx, fp, st);
-    case 128: return factor_level_nb<T, 4, 128>(L, E, mask, B, N, n, lidx, fp, st);
+    case 32: return factor_level_nb<T, 1, 32>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
+    case 64: return factor_level_nb<T, 2, 64>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
+    case 96: return factor_level_nb<T, 3, 96>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
+    case 128: return factor_level_nb<T, 4, 128>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
     default: break;
   }
   switch ((n + 31) / 32) {
-    case 1: return factor_level_nb<T, 1, 0>(L, E, mask, B, N, n, lidx, fp, st);
-    case 2: return factor_level_nb<T, 2, 0>(L, E, mask, B, N, n, lidx, fp, st);
-    case 3: return factor_level_nb<T, 3, 0>(L, E, mask, B, N, n, lidx, fp, st);
-    case 4: return factor_level_nb<T, 4, 0>(L, E, mask, B, N, n, lidx, fp, st);
+    case 1: return factor_level_nb<T, 1, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
+    case 2: return factor_level_nb<T, 2, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
+    case 3: return factor_level_nb<T, 3, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
+    case 4: return factor_level_nb<T, 4, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <typename T, int NB, int NF>
 static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
-                                  int step, hipStream_t st, bool apply_next) {
+                                  int step, hipStream_t st, bool apply_next, bool update = true) {
   const int nE = (int)off_diag_storage_len(N);
   const int mch = (m + kRhsChunk - 1) / kRhsChunk;
   if (B <= 0 || mch <= 0) return hipSuccess;
   kt_begin(kKcForward, st);
-  if (step >= 0) {
+  if (update && step >= 0) {
     const Level lv = level_info(N, step);
     const int items = N / (2 * lv.s);
     if (items > 0)
@@ -1212,23 +1220,40 @@ static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* m
 }
 
 template <typename T>
-hipError_t forward_step_blocked(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
-                                int step, hipStream_t st, bool apply_next) {
-  const bool a = apply_next;
+static hipError_t forward_step_parts(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
+                              int step, hipStream_t st, bool a, bool u) {
   switch (n) {
-    case 32: return forward_step_nb<T, 1, 32>(L, E, x, mask, B, N, n, m, step, st, a);
-    case 64: return forward_step_nb<T, 2, 64>(L, E, x, mask, B, N, n, m, step, st, a);
-    case 96: return forward_step_nb<T, 3, 96>(L, E, x, mask, B, N, n, m, step, st, a);
-    case 128: return forward_step_nb<T, 4, 128>(L, E, x, mask, B, N, n, m, step, st, a);
+    case 32: return forward_step_nb<T, 1, 32>(L, E, x, mask, B, N, n, m, step, st, a, u);
+    case 64: return forward_step_nb<T, 2, 64>(L, E, x, mask, B, N, n, m, step, st, a, u);
+    case 96: return forward_step_nb<T, 3, 96>(L, E, x, mask, B, N, n, m, step, st, a, u);
+    case 128: return forward_step_nb<T, 4, 128>(L, E, x, mask, B, N, n, m, step, st, a, u);
     default: break;
   }
   switch ((n + 31) / 32) {
-    case 1: return forward_step_nb<T, 1, 0>(L, E, x, mask, B, N, n, m, step, st, a);
-    case 2: return forward_step_nb<T, 2, 0>(L, E, x, mask, B, N, n, m, step, st, a);
-    case 3: return forward_step_nb<T, 3, 0>(L, E, x, mask, B, N, n, m, step, st, a);
-    case 4: return forward_step_nb<T, 4, 0>(L, E, x, mask, B, N, n, m, step, st, a);
+    case 1: return forward_step_nb<T, 1, 0>(L, E, x, mask, B, N, n, m, step, st, a, u);
+    case 2: return forward_step_nb<T, 2, 0>(L, E, x, mask, B, N, n, m, step, st, a, u);
+    case 3: return forward_step_nb<T, 3, 0>(L, E, x, mask, B, N, n, m, step, st, a, u);
+    case 4: return forward_step_nb<T, 4, 0>(L, E, x, mask, B, N, n, m, step, st, a, u);
     default: return hipErrorInvalidValue;
   }
+}
+
+template <typename T>
+hipError_t forward_step_blocked(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
+                                int step, hipStream_t st, bool apply_next) {
+  return forward_step_parts<T>(L, E, x, mask, B, N, n, m, step, st, apply_next, true);
+}
+
+template <typename T>
+hipError_t forward_update_blocked(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
+                                  int step, hipStream_t st) {
+  return forward_step_parts<T>(L, E, x, mask, B, N, n, m, step, st, false, true);
+}
+
+template <typename T>
+hipError_t forward_apply_blocked(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
+                                 int level, hipStream_t st) {
+  return forward_step_parts<T>(L, E, x, mask, B, N, n, m, level - 1, st, true, false);
 }
 
 template <typename T, int NB, int NF>
@@ -1268,14 +1293,76 @@ hipError_t backward_level_blocked(const T* L, const T* E, T* x, const uint8_t* m
   }
 }
 
+hipError_t overlap_create(SweepOverlap& ov) {
+  hipError_t e = hipStreamCreateWithFlags(&ov.side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ov.fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ov.join, hipEventDisableTiming);
+  for (int l = 0; l < kSweepMaxLevels && e == hipSuccess; ++l) {
+    e = hipEventCreateWithFlags(&ov.potrf[l], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ov.trsm[l], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) overlap_destroy(ov);
+  return e;
+}
+
+void overlap_destroy(SweepOverlap& ov) {
+  for (int l = 0; l < kSweepMaxLevels; ++l) {
+    if (ov.potrf[l]) (void)hipEventDestroy(ov.potrf[l]);
+    if (ov.trsm[l]) (void)hipEventDestroy(ov.trsm[l]);
+  }
+  if (ov.fork) (void)hipEventDestroy(ov.fork);
+  if (ov.join) (void)hipEventDestroy(ov.join);
+  if (ov.side) (void)hipStreamDestroy(ov.side);
+  ov = SweepOverlap();
+}
+
+// The reference's fused factor_and_solve likewise interleaves the forward
+// sweep into the factor's levels (block_tridiag_solver.py:926-1233).  Here
+// the sweep runs on a second stream: the inverses of level l follow that
+// level's potrf, its updates into the receiving blocks follow its trsm.
+// Both read only finished factor blocks and write only x, which the factor
+// does not touch.
+template <typename T>
+hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, const uint8_t* sweep_mask, int B, int N,
+                               int n, int m, bool f64_pivots, const SweepOverlap& ov, hipStream_t st) {
+  const int Lv = recursive_iterations(N);
+  hipError_t e = hipEventRecord(ov.fork, st);
+  if (e == hipSuccess) e = hipStreamWaitEvent(ov.side, ov.fork, 0);
+  for (int l = 0; l < Lv && e == hipSuccess; ++l) {
+    const bool last = l == Lv - 1;
+    e = factor_level_blocked<T>(L, E, factor_mask, B, N, n, l, f64_pivots, st, ov.potrf[l],
+                                last ? nullptr : ov.trsm[l]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ov.side, ov.potrf[l], 0);
+    if (e == hipSuccess) e = forward_apply_blocked<T>(L, E, x, sweep_mask, B, N, n, m, l, ov.side);
+    if (e == hipSuccess && !last) e = hipStreamWaitEvent(ov.side, ov.trsm[l], 0);
+    if (e == hipSuccess && !last) e = forward_update_blocked<T>(L, E, x, sweep_mask, B, N, n, m, l, ov.side);
+  }
+  if (e == hipSuccess) e = hipEventRecord(ov.join, ov.side);
+  if (e == hipSuccess) e = hipStreamWaitEvent(st, ov.join, 0);
+  return e;
+}
+
+template hipError_t factor_with_forward<float>(float*, float*, float*, const uint8_t*, const uint8_t*, int, int, int,
+                                               int, bool, const SweepOverlap&, hipStream_t);
+template hipError_t factor_with_forward<double>(double*, double*, double*, const uint8_t*, const uint8_t*, int, int,
+                                                int, int, bool, const SweepOverlap&, hipStream_t);
+
 template hipError_t factor_level_blocked<float>(float*, float*, const uint8_t*, int, int, int, int, bool,
-                                                hipStream_t);
+                                                hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t factor_level_blocked<double>(double*, double*, const uint8_t*, int, int, int, int, bool,
-                                                 hipStream_t);
+                                                 hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t forward_step_blocked<float>(const float*, const float*, float*, const uint8_t*, int, int, int, int,
                                                 int, hipStream_t, bool);
 template hipError_t forward_step_blocked<double>(const double*, const double*, double*, const uint8_t*, int, int, int,
                                                  int, int, hipStream_t, bool);
+template hipError_t forward_update_blocked<float>(const float*, const float*, float*, const uint8_t*, int, int, int,
+                                                  int, int, hipStream_t);
+template hipError_t forward_update_blocked<double>(const double*, const double*, double*, const uint8_t*, int, int,
+                                                   int, int, int, hipStream_t);
+template hipError_t forward_apply_blocked<float>(const float*, const float*, float*, const uint8_t*, int, int, int,
+                                                 int, int, hipStream_t);
+template hipError_t forward_apply_blocked<double>(const double*, const double*, double*, const uint8_t*, int, int,
+                                                  int, int, int, hipStream_t);
 template hipError_t backward_level_blocked<float>(const float*, const float*, float*, const uint8_t*, int, int, int,
                                                   int, int, hipStream_t);
 template hipError_t backward_level_blocked<double>(const double*, const double*, double*, const uint8_t*, int, int,

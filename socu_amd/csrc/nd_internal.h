@@ -32,9 +32,12 @@ hipError_t init_blocked_kernels();
 // is the last level), for block size n <= kMaxN.
 // f64_pivots (float storage only): the diagonal blocks are factored and
 // inverted in FP64 and rounded to FP32 on store.
+// after_potrf / after_trsm (optional) are recorded on st after the level's
+// potrf and trsm launches (the forward sweep can follow them on another
+// stream).
 template <typename T>
 hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, bool f64_pivots,
-                                hipStream_t st);
+                                hipStream_t st, hipEvent_t after_potrf = nullptr, hipEvent_t after_trsm = nullptr);
 
 // Forward-substitution step: step -1 applies level 0; step l >= 0 applies
 // level l's updates to every receiving block, then (unless apply_next is
@@ -42,6 +45,16 @@ hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, i
 template <typename T>
 hipError_t forward_step_blocked(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
                                 int step, hipStream_t st, bool apply_next = true);
+
+// The two halves of a forward step: level `step`'s updates into every
+// receiving block (needs that level's trsm) and the inverses of level
+// `level` (needs its potrf and every update into it).
+template <typename T>
+hipError_t forward_update_blocked(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
+                                  int step, hipStream_t st);
+template <typename T>
+hipError_t forward_apply_blocked(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
+                                 int level, hipStream_t st);
 
 // Backward-substitution kernels of one ND level.
 template <typename T>
@@ -54,6 +67,7 @@ hipError_t backward_level_blocked(const T* L, const T* E, T* x, const uint8_t* m
 // elements of the solve's type (one n-vector per contribution).
 constexpr int kSweepHeader = 4;             // ticket, status, 2 spare
 constexpr int kSweepSpinLimit = 1 << 21;    // polls per dependency wait
+constexpr int kSweepMaxLevels = 32;         // levels of any N < 2^31
 // Contribution slots of one system: two per receiving block and step,
 // sum over levels l of 2 * (N >> (l+1)).
 __host__ __device__ inline int sweep_slots_per_system(int N) {
@@ -73,6 +87,24 @@ hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, i
 // sweeps: where they measured faster (FP32, n > 96) unless the environment
 // variable SOCU_SWEEP is "level" (never) or "persistent" (always).
 bool sweep_persistent_enabled(int n, int dsize);
+
+// A side stream and per-level events that let the forward sweep of a
+// factor_and_solve run alongside the factor: forward step l waits for level
+// l's trsm, the inverses of level l for its potrf (created with a plan,
+// destroyed with it).
+struct SweepOverlap {
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t potrf[kSweepMaxLevels] = {};
+  hipEvent_t trsm[kSweepMaxLevels] = {};
+};
+hipError_t overlap_create(SweepOverlap& ov);
+void overlap_destroy(SweepOverlap& ov);
+// Factor levels on st with the forward sweep of x on ov.side, joined back
+// into st at the end (the backward sweep is the caller's).
+template <typename T>
+hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, const uint8_t* sweep_mask, int B, int N,
+                               int n, int m, bool f64_pivots, const SweepOverlap& ov, hipStream_t st);
 
 // Mixed-precision factor_and_solve with FP64 refinement (nd_refine.hip).
 struct MixedPlan {
@@ -95,6 +127,7 @@ struct MixedPlan {
   int nblk_init = 1;
   int* sweep_ws = nullptr;     // persistent sweeps (m == 1), else NULL
   float* sweep_slots = nullptr;
+  SweepOverlap overlap;        // first correction solve's forward sweep alongside the factor
 };
 hipError_t init_refine_kernels();
 

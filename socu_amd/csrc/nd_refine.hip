@@ -338,7 +338,7 @@ hipError_t mixed_alloc(MixedPlan& p) {
     if ((e = hipMemset(p.sweep_ws, 0, sizeof(int) * sweep_ws_ints(p.B, p.N))) != hipSuccess) return e;
     if ((e = hipMalloc(&p.sweep_slots, sizeof(float) * sweep_slot_elems(p.B, p.N, p.n))) != hipSuccess) return e;
   }
-  return hipSuccess;
+  return overlap_create(p.overlap);
 }
 
 void mixed_free(MixedPlan& p) {
@@ -350,6 +350,7 @@ void mixed_free(MixedPlan& p) {
   p.active = nullptr;
   p.sweep_ws = nullptr;
   p.sweep_slots = nullptr;
+  overlap_destroy(p.overlap);
 }
 
 hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
@@ -379,23 +380,36 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
                          Ep, lc, ec, es, p.mask);
   }
   const int Lv = recursive_iterations(N);
-  for (int l = 0; l < Lv; ++l)
-    if ((e = factor_level_blocked<float>(p.L32, p.E32, p.mask, B, N, n, l, p.f64_pivots, st)) != hipSuccess) return e;
+  // x <- 0 and the first right-hand side d32 <- b: independent of the factor
   hipLaunchKernelGGL(k_refine_init, dim3(p.nblk_init, B), dim3(kThreads), 0, st, p.x, p.b64, p.d32, p.partial, xcount,
                      p.nblk_init, p.mask);
   hipLaunchKernelGGL(k_refine_start, dim3(B), dim3(kThreads), 0, st, p.state, p.active, p.partial, p.nblk_init,
                      p.mask);
+  const bool overlap = p.overlap.side != nullptr;
+  if (overlap) {
+    // the first correction solve's forward sweep runs alongside the factor
+    if ((e = factor_with_forward<float>(p.L32, p.E32, p.d32, p.mask, p.active, B, N, n, m, p.f64_pivots, p.overlap,
+                                        st)) != hipSuccess)
+      return e;
+  } else {
+    for (int l = 0; l < Lv; ++l)
+      if ((e = factor_level_blocked<float>(p.L32, p.E32, p.mask, B, N, n, l, p.f64_pivots, st)) != hipSuccess)
+        return e;
+  }
   const size_t rlds = mixed_residual_lds(n, m);
   for (int it = 0; it < p.max_iter; ++it) {
+    const bool fwd = !(overlap && it == 0);  // done alongside the factor
     if (p.sweep_ws != nullptr) {
-      // both sweeps in one persistent launch; converged systems are skipped
-      if ((e = sweep_persistent<float>(p.L32, p.E32, p.d32, p.active, B, N, n, true, true, p.sweep_ws, p.sweep_slots,
+      // the upper levels of both sweeps in one persistent launch; converged
+      // systems are skipped
+      if ((e = sweep_persistent<float>(p.L32, p.E32, p.d32, p.active, B, N, n, fwd, true, p.sweep_ws, p.sweep_slots,
                                        st)) != hipSuccess)
         return e;
     } else {
-      for (int step = -1; step < Lv - 1; ++step)
-        if ((e = forward_step_blocked<float>(p.L32, p.E32, p.d32, p.active, B, N, n, m, step, st)) != hipSuccess)
-          return e;
+      if (fwd)
+        for (int step = -1; step < Lv - 1; ++step)
+          if ((e = forward_step_blocked<float>(p.L32, p.E32, p.d32, p.active, B, N, n, m, step, st)) != hipSuccess)
+            return e;
       for (int l = Lv - 1; l >= 0; --l)
         if ((e = backward_level_blocked<float>(p.L32, p.E32, p.d32, p.active, B, N, n, m, l, st)) != hipSuccess)
           return e;
