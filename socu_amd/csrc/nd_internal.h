@@ -127,6 +127,10 @@ struct MixedPlan {
   int nblk_init = 1;
   int* sweep_ws = nullptr;     // persistent sweeps (m == 1), else NULL
   float* sweep_slots = nullptr;
+  double* res_w = nullptr;     // one-RHS residual pieces (m == 1), else NULL
+  double* res_v = nullptr;
+  double* res_c = nullptr;
+  int* res_count = nullptr;    // per system: workgroups of k_resid1_finish done
   SweepOverlap overlap;        // first correction solve's forward sweep alongside the factor
 };
 hipError_t init_refine_kernels();

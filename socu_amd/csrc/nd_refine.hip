@@ -292,6 +292,155 @@ __global__ __launch_bounds__(kThreads) void k_refine_check(double* __restrict__ 
   }
 }
 
+// ---------------------------------------------------------------------------
+// One right-hand side (m == 1): update, residual and convergence test in two
+// launches that read tril(L_i) and E_i once each (the general kernel above
+// reads both triangles of L_i and every E block twice).  With y = x + d:
+//   k_resid1_blocks, one workgroup per (block row i, slice s of 32 rows):
+//     rows r of the slice:  W_i[r]     = b_i[r] - sum_{k<=r} L_i[r][k] y_i[k]
+//                           V_{i+1}[r] = sum_k E_i[r][k] y_i[k]   (E_i = block (i+1, i))
+//     every column k, summed over the slice's rows:
+//                           C_{i,s}[k] = sum_{r>k} L_i[r][k] y_i[r] + sum_r E_i[r][k] y_{i+1}[r]
+//   k_resid1_finish: x += d, r_i = W_i - sum_s C_{i,s} - V_i, r32 <- r and the
+//     ||r||^2 partials; the last workgroup of a system to finish sums the
+//     partials in a fixed order and applies the convergence test of
+//     k_refine_check, so the result does not depend on the arrival order.
+// Lanes: 16 per row (each load instruction reads 128 contiguous bytes of four
+// rows), row r = 32 s + (tid >> 4) + 16 g, column k = (tid & 15) + 16 j.
+__global__ __launch_bounds__(kThreads) void k_resid1_blocks(const double* __restrict__ L, const double* __restrict__ E,
+                                                           const double* __restrict__ x, const float* __restrict__ d,
+                                                           const double* __restrict__ b_in, double* __restrict__ W,
+                                                           double* __restrict__ V, double* __restrict__ C,
+                                                           const double* __restrict__ state, int N, int n, int nE,
+                                                           int nsl) {
+  const int b = blockIdx.y;
+  if (refine_done(state, b)) return;
+  const int i = blockIdx.x / nsl, s = blockIdx.x - i * nsl;
+  const size_t nn = (size_t)n * n;
+  __shared__ double yc[kMaxN], yn[kMaxN];
+  __shared__ double redc[16][kMaxN];
+  __shared__ double redr[2][32][17];
+  const double* xb = x + (size_t)b * N * n;
+  const float* db = d + (size_t)b * N * n;
+  const bool hasE = i + 1 < N;
+  for (int q = threadIdx.x; q < 2 * n; q += kThreads) {
+    const int w = q >= n ? 1 : 0, e = q - w * n, row = i + w;
+    const double y = (row < N) ? xb[(size_t)row * n + e] + (double)db[(size_t)row * n + e] : 0.0;
+    (w ? yn : yc)[e] = y;
+  }
+  const int t = threadIdx.x, rr = t >> 4, rs = t & 15;
+  const double* Li = L + ((size_t)b * N + i) * nn;
+  const double* Ei = E + (size_t)b * nE * nn + (size_t)i * nn;  // read only when hasE
+  double ev[2][8], lv[2][8];
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    const int r = 32 * s + rr + 16 * g;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = rs + 16 * j;
+      const bool in = r < n && k < n;
+      ev[g][j] = (hasE && in) ? Ei[(size_t)r * n + k] : 0.0;
+      lv[g][j] = (in && 16 * j <= r) ? Li[(size_t)r * n + k] : 0.0;  // skips segments above the diagonal
+    }
+  }
+  __syncthreads();  // yc, yn
+  double ycol[8], cacc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k = rs + 16 * j;
+    ycol[j] = (k < n) ? yc[k] : 0.0;
+    cacc[j] = 0.0;
+  }
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    const int q = rr + 16 * g, r = 32 * s + q;
+    double ra = 0.0, va = 0.0;
+    if (r < n) {
+      const double yr = yc[r], ynr = yn[r];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = rs + 16 * j;
+        va = fma(ev[g][j], ycol[j], va);
+        cacc[j] = fma(ev[g][j], ynr, cacc[j]);
+        ra = fma((k <= r) ? lv[g][j] : 0.0, ycol[j], ra);
+        cacc[j] = fma((k < r) ? lv[g][j] : 0.0, yr, cacc[j]);
+      }
+    }
+    redr[0][q][rs] = ra;
+    redr[1][q][rs] = va;
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) redc[rr][rs + 16 * j] = cacc[j];
+  __syncthreads();
+  const size_t row0 = ((size_t)b * N + i) * n;
+  if (t < 64) {
+    const int which = t >> 5, q = t & 31, r = 32 * s + q;
+    double sum = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) sum += redr[which][q][u];
+    if (r < n) {
+      if (which == 0) W[row0 + r] = b_in[row0 + r] - sum;
+      else if (hasE) V[row0 + n + r] = sum;
+    }
+  }
+  for (int k = t; k < n; k += kThreads) {
+    double sum = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) sum += redc[u][k];
+    C[(((size_t)b * N + i) * nsl + s) * n + k] = sum;
+  }
+}
+
+// dr holds the correction d on entry and the FP32 residual on exit.
+__global__ __launch_bounds__(kThreads) void k_resid1_finish(double* __restrict__ x, float* __restrict__ dr,
+                                                           const double* __restrict__ W, const double* __restrict__ V,
+                                                           const double* __restrict__ C, double* __restrict__ partial, double* __restrict__ state,
+                                                           uint8_t* __restrict__ active, int* __restrict__ counter,
+                                                           int N, int n, int nsl, double tol2) {
+  const int b = blockIdx.y;
+  if (refine_done(state, b)) return;
+  const size_t count = (size_t)N * n;
+  __shared__ double red[kThreads];
+  __shared__ int last;
+  double sq = 0.0;
+  for (size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x; e < count; e += (size_t)gridDim.x * kThreads) {
+    const size_t gi = (size_t)b * count + e;
+    const int i = (int)(e / n), a = (int)(e - (size_t)i * n);
+    double r = W[gi];
+    const double* Ci = C + ((size_t)b * N + i) * nsl * n + a;
+    for (int q = 0; q < nsl; ++q) r -= Ci[(size_t)q * n];
+    if (i > 0) r -= V[gi];
+    x[gi] += (double)dr[gi];
+    dr[gi] = (float)r;
+    sq += r * r;
+  }
+  red[threadIdx.x] = sq;
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[(size_t)b * gridDim.x + blockIdx.x] = red[0];
+    // release of the partial, acquire of everyone else's for the last one
+    const int old = __hip_atomic_fetch_add(counter + b, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = (old == (int)gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  const double s = block_sum(partial + (size_t)b * gridDim.x, gridDim.x, red);
+  if (threadIdx.x == 0) {
+    state[4 * b + 1] = s;
+    state[4 * b + 3] += 1.0;
+    if (s <= tol2 * state[4 * b + 0] || !(s == s)) {
+      state[4 * b + 2] = 1.0;
+      active[b] = 0;
+    }
+    __hip_atomic_store(counter + b, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 }  // namespace socu
 
 // ---------------------------------------------------------------------------
@@ -333,6 +482,14 @@ hipError_t mixed_alloc(MixedPlan& p) {
   const int np = nres > p.nblk_init ? nres : p.nblk_init;
   if ((e = hipMalloc(&p.partial, sizeof(double) * p.B * (size_t)np)) != hipSuccess) return e;
   if ((e = hipMalloc(&p.active, (size_t)p.B)) != hipSuccess) return e;
+  if (p.m == 1) {
+    const size_t rows = (size_t)p.B * p.N * p.n;
+    if ((e = hipMalloc(&p.res_w, sizeof(double) * rows)) != hipSuccess) return e;
+    if ((e = hipMalloc(&p.res_v, sizeof(double) * rows)) != hipSuccess) return e;
+    if ((e = hipMalloc(&p.res_c, sizeof(double) * rows * residual_slices(p.n))) != hipSuccess) return e;
+    if ((e = hipMalloc(&p.res_count, sizeof(int) * p.B)) != hipSuccess) return e;
+    if ((e = hipMemset(p.res_count, 0, sizeof(int) * p.B)) != hipSuccess) return e;
+  }
   if (p.m == 1 && sweep_persistent_enabled(p.n, (int)sizeof(float))) {
     if ((e = hipMalloc(&p.sweep_ws, sizeof(int) * sweep_ws_ints(p.B, p.N))) != hipSuccess) return e;
     if ((e = hipMemset(p.sweep_ws, 0, sizeof(int) * sweep_ws_ints(p.B, p.N))) != hipSuccess) return e;
@@ -342,7 +499,8 @@ hipError_t mixed_alloc(MixedPlan& p) {
 }
 
 void mixed_free(MixedPlan& p) {
-  void* ptrs[] = {p.L32, p.E32, p.d32, p.b64, p.state, p.partial, p.active, p.sweep_ws, p.sweep_slots};
+  void* ptrs[] = {p.L32,    p.E32,         p.d32,   p.b64,   p.state, p.partial,  p.active,
+                  p.sweep_ws, p.sweep_slots, p.res_w, p.res_v, p.res_c, p.res_count};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   p.L32 = p.E32 = p.d32 = nullptr;
@@ -350,6 +508,8 @@ void mixed_free(MixedPlan& p) {
   p.active = nullptr;
   p.sweep_ws = nullptr;
   p.sweep_slots = nullptr;
+  p.res_w = p.res_v = p.res_c = nullptr;
+  p.res_count = nullptr;
   overlap_destroy(p.overlap);
 }
 
@@ -415,6 +575,16 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
           return e;
     }
     kt_begin(kKcRefine, st);
+    if (p.res_w != nullptr) {
+      const int nsl = residual_slices(n);
+      hipLaunchKernelGGL(k_resid1_blocks, dim3(N * nsl, B), dim3(kThreads), 0, st, p.L, p.E, p.x, p.d32, p.b64,
+                         p.res_w, p.res_v, p.res_c, p.state, N, n, nE, nsl);
+      hipLaunchKernelGGL(k_resid1_finish, dim3(refine_blocks(xcount), B), dim3(kThreads), 0, st, p.x, p.d32,
+                         p.res_w, p.res_v, p.res_c, p.partial, p.state, p.active, p.res_count, N, n, nsl,
+                         p.tol * p.tol);
+      kt_end(kKcRefine, st);
+      continue;
+    }
     hipLaunchKernelGGL(k_refine_update, dim3(refine_blocks(xcount), B), dim3(kThreads), 0, st, p.x, p.d32, xcount,
                        p.state);
     const int nsl = residual_slices(n);
