@@ -52,6 +52,9 @@ int check_dims(const void* L, const void* E, const void* x, bool need_x, int B, 
     if (need_x && x == nullptr) return fail(SOCU_EINVAL, "x is NULL");
   }
   if (B > 65535) return fail(SOCU_EUNSUPPORTED, "batch dimension %d exceeds 65535", B);
+  // per-system offsets inside the kernels are 32-bit
+  if ((size_t)N * n * n > 0x7fffffffu)
+    return fail(SOCU_EUNSUPPORTED, "N*n*n = %zu exceeds 2^31 - 1 elements per system", (size_t)N * n * n);
   return SOCU_OK;
 }
 

@@ -53,28 +53,72 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
   T* col = dinv + NP;
   const int tid = threadIdx.x, wave = wave_id(), l = lane_id();
 
-  // The block is read as one contiguous n*n run (coalesced, unconditional
-  // loads from clamped addresses) and scattered into the padded LDS square.
-  constexpr int PER = NP * NP / kThreads;
   const int nn = n * n;
-  T v[PER];
+  if constexpr (NF == NP) {
+    // Unpadded block: only the lower tiles are read (every strictly upper
+    // tile is written below before it is read).  Wave 0 reads diagonal tile
+    // (0, 0) alone and starts its pivot chain at once; waves 1..3, idle in
+    // that step, read the other lower tiles meanwhile (the step's barrier
+    // publishes them).  Lane l reads row 2q + l/32, column l%32 of a tile.
+    const int lr = l >> 5, lc = l & 31;
+    if (wave == 0) {
+      TS v[16];
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = tid + e * kThreads;
-    v[e] = (T)A[min(idx, nn - 1)];
-  }
-  for (int idx = tid; idx < NP * NP; idx += kThreads) {
-    const int r = idx / NP, c = idx - r * NP;
-    S[r * P + c] = (r == c) ? T(1) : T(0);
-  }
-  __syncthreads();
+      for (int q = 0; q < 16; ++q) v[q] = A[(2 * q + lr) * NF + lc];
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = tid + e * kThreads;
-    const int r = idx / n, c = idx - r * n;
-    if (idx < nn) S[r * P + c] = (c <= r) ? v[e] : T(0);
+      for (int q = 0; q < 16; ++q) {
+        const int r = 2 * q + lr;
+        S[r * P + lc] = (lc <= r) ? (T)v[q] : T(0);
+      }
+    } else {
+      constexpr int NT = NB * (NB + 1) / 2 - 1;  // lower tiles other than (0, 0)
+      constexpr int PW = NT > 0 ? (NT + 2) / 3 : 1;
+      TS v[PW][16];
+#pragma unroll
+      for (int k = 0; k < PW; ++k) {
+        int ti = 1, tj = (wave - 1) + 3 * k;
+        while (tj > ti) { tj -= ti + 1; ++ti; }
+        if (ti < NB) {
+#pragma unroll
+          for (int q = 0; q < 16; ++q) v[k][q] = A[(32 * ti + 2 * q + lr) * NF + 32 * tj + lc];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < PW; ++k) {
+        int ti = 1, tj = (wave - 1) + 3 * k;
+        while (tj > ti) { tj -= ti + 1; ++ti; }
+        if (ti < NB) {
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            const int r = 2 * q + lr;
+            S[(32 * ti + r) * P + 32 * tj + lc] = (ti != tj || lc <= r) ? (T)v[k][q] : T(0);
+          }
+        }
+      }
+    }
+  } else {
+    // The block is read as one contiguous n*n run (coalesced, unconditional
+    // loads from clamped addresses) and scattered into the padded LDS square.
+    constexpr int PER = NP * NP / kThreads;
+    T v[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int idx = tid + e * kThreads;
+      v[e] = (T)A[min(idx, nn - 1)];
+    }
+    for (int idx = tid; idx < NP * NP; idx += kThreads) {
+      const int r = idx / NP, c = idx - r * NP;
+      S[r * P + c] = (r == c) ? T(1) : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int idx = tid + e * kThreads;
+      const int r = idx / n, c = idx - r * n;
+      if (idx < nn) S[r * P + c] = (c <= r) ? v[e] : T(0);
+    }
+    __syncthreads();
   }
-  __syncthreads();
 
   // Off-diagonal tiles of the inverse:
   //   Linv_ij = -Dinv_i S_ij,  S_ij = sum_{k=j}^{i-1} L_ik Linv_kj  (j < i),

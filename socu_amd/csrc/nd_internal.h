@@ -130,7 +130,6 @@ struct MixedPlan {
   double* res_w = nullptr;     // one-RHS residual pieces (m == 1), else NULL
   double* res_v = nullptr;
   double* res_c = nullptr;
-  int* res_count = nullptr;    // per system: workgroups of k_resid1_finish done
   SweepOverlap overlap;        // first correction solve's forward sweep alongside the factor
 };
 hipError_t init_refine_kernels();

@@ -22,14 +22,19 @@ namespace socu {
 // ---------------------------------------------------------------------------
 
 // FP32 working copy of Psi in one launch: per batch entry
-//   L32 <- L (lc elements), E32[0, ec) <- E (the tridiagonal blocks),
-//   E32[ec, es) <- 0 (fill-in region).
+//   tril(L32) <- tril(L) (lc elements), E32[0, ec) <- E (the tridiagonal
+//   blocks), E32[ec, es) <- 0 (fill-in region).
+// The strict upper triangle of L32 is left as it is: the factor kernels read
+// only tril of a diagonal block and then store its inverse there (the
+// reference also reads only tril(L), potrf.py:101-103), so half of L is
+// neither read nor written.
 // VEC: four elements per step (two 16-byte loads, one 16-byte store), for
-// lc, ec, es multiples of 4 and 16-byte aligned arrays.
+// lc, ec, es multiples of 4 and 16-byte aligned arrays; a step is skipped
+// when its four elements lie in one row, strictly above the diagonal.
 template <bool VEC>
 __global__ __launch_bounds__(kThreads) void k_psi_to_f32(float* __restrict__ L32, float* __restrict__ E32,
                                                         const double* __restrict__ L, const double* __restrict__ E,
-                                                        size_t lc, size_t ec, size_t es,
+                                                        size_t lc, size_t ec, size_t es, int n,
                                                         const uint8_t* __restrict__ mask) {
   const int b = blockIdx.y;
   if (masked_out(mask, b)) return;
@@ -42,6 +47,8 @@ __global__ __launch_bounds__(kThreads) void k_psi_to_f32(float* __restrict__ L32
   for (size_t u = (size_t)blockIdx.x * kThreads + threadIdx.x; u < total; u += (size_t)gridDim.x * kThreads) {
     if (VEC) {
       if (u < nl) {
+        const int e = (int)((unsigned)(4 * u) % (unsigned)(n * n)), r = e / n, k = e - r * n;
+        if (k > r && k + 3 < n) continue;
         const double2 a = reinterpret_cast<const double2*>(Lb)[2 * u];
         const double2 c = reinterpret_cast<const double2*>(Lb)[2 * u + 1];
         reinterpret_cast<float4*>(L32b)[u] = make_float4((float)a.x, (float)a.y, (float)c.x, (float)c.y);
@@ -54,7 +61,10 @@ __global__ __launch_bounds__(kThreads) void k_psi_to_f32(float* __restrict__ L32
         reinterpret_cast<float4*>(E32b)[u - nl] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     } else {
-      if (u < nl) L32b[u] = (float)Lb[u];
+      if (u < nl) {
+        const int e = (int)((unsigned)u % (unsigned)(n * n)), r = e / n;
+        if (e - r * n <= r) L32b[u] = (float)Lb[u];
+      }
       else if (u - nl < ne) E32b[u - nl] = (float)Eb[u - nl];
       else E32b[u - nl] = 0.f;
     }
@@ -302,9 +312,10 @@ __global__ __launch_bounds__(kThreads) void k_refine_check(double* __restrict__ 
 //     every column k, summed over the slice's rows:
 //                           C_{i,s}[k] = sum_{r>k} L_i[r][k] y_i[r] + sum_r E_i[r][k] y_{i+1}[r]
 //   k_resid1_finish: x += d, r_i = W_i - sum_s C_{i,s} - V_i, r32 <- r and the
-//     ||r||^2 partials; the last workgroup of a system to finish sums the
-//     partials in a fixed order and applies the convergence test of
-//     k_refine_check, so the result does not depend on the arrival order.
+//     ||r||^2 partials (k_refine_check then sums them and applies the
+//     convergence test; a last-workgroup-done counter in place of that launch
+//     measured slower: its release fences write the L2 back in every
+//     workgroup).
 // Lanes: 16 per row (each load instruction reads 128 contiguous bytes of four
 // rows), row r = 32 s + (tid >> 4) + 16 g, column k = (tid & 15) + 16 j.
 __global__ __launch_bounds__(kThreads) void k_resid1_blocks(const double* __restrict__ L, const double* __restrict__ E,
@@ -394,51 +405,30 @@ __global__ __launch_bounds__(kThreads) void k_resid1_blocks(const double* __rest
 // dr holds the correction d on entry and the FP32 residual on exit.
 __global__ __launch_bounds__(kThreads) void k_resid1_finish(double* __restrict__ x, float* __restrict__ dr,
                                                            const double* __restrict__ W, const double* __restrict__ V,
-                                                           const double* __restrict__ C, double* __restrict__ partial, double* __restrict__ state,
-                                                           uint8_t* __restrict__ active, int* __restrict__ counter,
-                                                           int N, int n, int nsl, double tol2) {
+                                                           const double* __restrict__ C, double* __restrict__ partial,
+                                                           const double* __restrict__ state, int N, int n, int nsl) {
   const int b = blockIdx.y;
   if (refine_done(state, b)) return;
-  const size_t count = (size_t)N * n;
-  __shared__ double red[kThreads];
-  __shared__ int last;
+  const int count = N * n;  // < 2^31 (checked by the host)
+  __shared__ double red[kThreads / 64];
+  const size_t base = (size_t)b * count;
   double sq = 0.0;
-  for (size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x; e < count; e += (size_t)gridDim.x * kThreads) {
-    const size_t gi = (size_t)b * count + e;
-    const int i = (int)(e / n), a = (int)(e - (size_t)i * n);
+  for (int e = blockIdx.x * kThreads + threadIdx.x; e < count; e += gridDim.x * kThreads) {
+    const size_t gi = base + e;
+    const int i = e / n, a = e - i * n;
     double r = W[gi];
-    const double* Ci = C + ((size_t)b * N + i) * nsl * n + a;
-    for (int q = 0; q < nsl; ++q) r -= Ci[(size_t)q * n];
+    const double* Ci = C + (((size_t)b * N + i) * nsl) * n + a;
+    for (int q = 0; q < nsl; ++q) r -= Ci[q * n];
     if (i > 0) r -= V[gi];
     x[gi] += (double)dr[gi];
     dr[gi] = (float)r;
     sq += r * r;
   }
-  red[threadIdx.x] = sq;
+  // wavefront sums, then the four of the workgroup in a fixed order
+  for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
   __syncthreads();
-  for (int o = kThreads / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    partial[(size_t)b * gridDim.x + blockIdx.x] = red[0];
-    // release of the partial, acquire of everyone else's for the last one
-    const int old = __hip_atomic_fetch_add(counter + b, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (old == (int)gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!last) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  const double s = block_sum(partial + (size_t)b * gridDim.x, gridDim.x, red);
-  if (threadIdx.x == 0) {
-    state[4 * b + 1] = s;
-    state[4 * b + 3] += 1.0;
-    if (s <= tol2 * state[4 * b + 0] || !(s == s)) {
-      state[4 * b + 2] = 1.0;
-      active[b] = 0;
-    }
-    __hip_atomic_store(counter + b, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (threadIdx.x == 0) partial[(size_t)b * gridDim.x + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 }  // namespace socu
@@ -460,6 +450,7 @@ size_t mixed_residual_lds(int n, int m) { return sizeof(double) * 3 * (size_t)n 
 constexpr size_t kResidualLdsMax = 140 * 1024;
 
 static int residual_slices(int n) { return (n + kResRows - 1) / kResRows; }
+
 
 hipError_t init_refine_kernels() {
   return hipFuncSetAttribute((const void*)k_residual, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -487,8 +478,6 @@ hipError_t mixed_alloc(MixedPlan& p) {
     if ((e = hipMalloc(&p.res_w, sizeof(double) * rows)) != hipSuccess) return e;
     if ((e = hipMalloc(&p.res_v, sizeof(double) * rows)) != hipSuccess) return e;
     if ((e = hipMalloc(&p.res_c, sizeof(double) * rows * residual_slices(p.n))) != hipSuccess) return e;
-    if ((e = hipMalloc(&p.res_count, sizeof(int) * p.B)) != hipSuccess) return e;
-    if ((e = hipMemset(p.res_count, 0, sizeof(int) * p.B)) != hipSuccess) return e;
   }
   if (p.m == 1 && sweep_persistent_enabled(p.n, (int)sizeof(float))) {
     if ((e = hipMalloc(&p.sweep_ws, sizeof(int) * sweep_ws_ints(p.B, p.N))) != hipSuccess) return e;
@@ -500,7 +489,7 @@ hipError_t mixed_alloc(MixedPlan& p) {
 
 void mixed_free(MixedPlan& p) {
   void* ptrs[] = {p.L32,    p.E32,         p.d32,   p.b64,   p.state, p.partial,  p.active,
-                  p.sweep_ws, p.sweep_slots, p.res_w, p.res_v, p.res_c, p.res_count};
+                  p.sweep_ws, p.sweep_slots, p.res_w, p.res_v, p.res_c};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   p.L32 = p.E32 = p.d32 = nullptr;
@@ -509,7 +498,6 @@ void mixed_free(MixedPlan& p) {
   p.sweep_ws = nullptr;
   p.sweep_slots = nullptr;
   p.res_w = p.res_v = p.res_c = nullptr;
-  p.res_count = nullptr;
   overlap_destroy(p.overlap);
 }
 
@@ -534,10 +522,10 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
     float* E32p = N > 1 ? p.E32 : p.L32;
     if (vec)
       hipLaunchKernelGGL((k_psi_to_f32<true>), dim3((unsigned)blocks, B), dim3(kThreads), 0, st, p.L32, E32p, p.L, Ep,
-                         lc, ec, es, p.mask);
+                         lc, ec, es, n, p.mask);
     else
       hipLaunchKernelGGL((k_psi_to_f32<false>), dim3((unsigned)blocks, B), dim3(kThreads), 0, st, p.L32, E32p, p.L,
-                         Ep, lc, ec, es, p.mask);
+                         Ep, lc, ec, es, n, p.mask);
   }
   const int Lv = recursive_iterations(N);
   // x <- 0 and the first right-hand side d32 <- b: independent of the factor
@@ -579,8 +567,10 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
       const int nsl = residual_slices(n);
       hipLaunchKernelGGL(k_resid1_blocks, dim3(N * nsl, B), dim3(kThreads), 0, st, p.L, p.E, p.x, p.d32, p.b64,
                          p.res_w, p.res_v, p.res_c, p.state, N, n, nE, nsl);
-      hipLaunchKernelGGL(k_resid1_finish, dim3(refine_blocks(xcount), B), dim3(kThreads), 0, st, p.x, p.d32,
-                         p.res_w, p.res_v, p.res_c, p.partial, p.state, p.active, p.res_count, N, n, nsl,
+      const int nb2 = refine_blocks(xcount);
+      hipLaunchKernelGGL(k_resid1_finish, dim3(nb2, B), dim3(kThreads), 0, st, p.x, p.d32, p.res_w, p.res_v,
+                         p.res_c, p.partial, p.state, N, n, nsl);
+      hipLaunchKernelGGL(k_refine_check, dim3(B), dim3(kThreads), 0, st, p.state, p.active, p.partial, nb2,
                          p.tol * p.tol);
       kt_end(kKcRefine, st);
       continue;
