@@ -157,6 +157,17 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
     acc_foreach<T>(inv, [&](int r, int c, T v) { Uji[c * P + r] = -v; });
   };
 
+  // 32x32 tile (ti, tj) of S written back to the same tile of A (unpadded
+  // blocks only): lane l stores row 2q + l/32, column l%32.
+  auto store_tile = [&](int ti, int tj) {
+    const int lr = l >> 5, lc = l & 31;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int r = 32 * ti + 2 * q + lr, c = 32 * tj + lc;
+      A[r * n + c] = (TS)S[r * P + c];
+    }
+  };
+
   // tile (gi, gj) -= L_{gi,kb} L_{gj,kb}^T
   auto trail = [&](int kb, int gi, int gj) {
     typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
@@ -187,6 +198,19 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
           int ti = 0, q = t;
           while (q > ti) { q -= ti + 1; ++ti; }
           trail(kb - 1, kb + ti, kb + q);
+        }
+      }
+      if constexpr (NF == NP) {
+        // Write-back of the tiles final since the end of step kb - 1 (the
+        // factor's tile column kb - 1 with the inverse of diagonal tile
+        // kb - 1, and the inverse blocks Linv_{kb-2, j}^T), while wave 0 is in
+        // its pivot chain.
+        if (kb > 0) {
+          const int nlow = NB - (kb - 1), nup = kb >= 2 ? kb - 2 : 0;
+          for (int t = wave - 1; t < nlow + nup; t += kThreads / 64 - 1) {
+            if (t < nlow) store_tile(kb - 1 + t, kb - 1);
+            else store_tile(t - nlow, kb - 2);
+          }
         }
       }
     }
@@ -220,13 +244,24 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
     for (int j = wave - 1; j < NB - 1; j += kThreads / 64 - 1) inv_finish(NB - 1, j);
   __syncthreads();
 
-  // Whole block written back as one contiguous run: tril(L) is the factor;
-  // the strict upper triangle, which is not part of the factor, carries the
-  // transposed inverse of the diagonal block (U[k][r] = Linv[r][k], k < r)
-  // for the trsm and solve kernels.
-  for (int idx = tid; idx < nn; idx += kThreads) {
-    const int r = idx / n, c = idx - r * n;
-    A[idx] = (TS)S[r * P + c];
+  // Write-back: tril(L) is the factor; the strict upper triangle, which is
+  // not part of the factor, carries the transposed inverse of the diagonal
+  // block (U[k][r] = Linv[r][k], k < r) for the trsm and solve kernels.
+  if constexpr (NF == NP) {
+    // what the pivot steps have not written yet: diagonal tile NB-1 and the
+    // inverse blocks Linv_{NB-2, j}^T, Linv_{NB-1, j}^T
+    const int n2 = NB >= 2 ? NB - 2 : 0, cnt = 1 + n2 + (NB - 1);
+    for (int t = wave; t < cnt; t += kThreads / 64) {
+      if (t == 0) store_tile(NB - 1, NB - 1);
+      else if (t - 1 < n2) store_tile(t - 1, NB - 2);
+      else store_tile(t - 1 - n2, NB - 1);
+    }
+  } else {
+    // one contiguous run
+    for (int idx = tid; idx < nn; idx += kThreads) {
+      const int r = idx / n, c = idx - r * n;
+      A[idx] = (TS)S[r * P + c];
+    }
   }
 }
 
