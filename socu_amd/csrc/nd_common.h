@@ -66,7 +66,10 @@ __host__ inline Level level_info(int N, int l) {
   return v;
 }
 
-__device__ __forceinline__ float rsqrt_t(float x) { return rsqrtf(x); }
+// v_rsq_f32 directly: rsqrtf adds a rescaling for denormal arguments around
+// the same instruction (identical results for normal arguments); a pivot
+// below 2^-126 means a numerically singular block anyway.
+__device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsqrt_t(double x) { return rsqrt(x); }
 __device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
 __device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }

@@ -150,7 +150,13 @@ __device__ __forceinline__ void acc16_foreach(typename Mfma<T>::acc_t& acc, F f)
 // k+1 updated by step k).  Columns k, k+1 come from the LDS buffer (k/2) % 2
 // (LDS operations of one wavefront complete in issue order, so no barrier is
 // needed).  Look-ahead: entries k+2, k+3 are updated first and published as
-// the next pair's columns before the remaining updates.
+// the next pair's columns, and the next pair's column reads are issued right
+// after (a scheduling barrier keeps them there), so that their LDS latency
+// runs under this pair's remaining updates instead of the next pivot chain.
+// The coefficients are not masked for row lanes at or above the pair: those
+// lanes only change entries of their strict upper triangle, which nothing
+// reads (not the column buffers, the pivots, nor the store), and their own
+// pivot entries are replaced explicitly.
 template <typename T>
 __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) {
   const int lane = lane_id();
@@ -172,18 +178,15 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
     colv[r] = u[0];
     colv[32 + r] = u[1];
   }
+  T c0[32], c1[32];
+#pragma unroll
+  for (int q = 2; q < 32; ++q) {
+    c0[q] = colv[q];
+    c1[q] = colv[32 + q];
+  }
 #pragma unroll
   for (int k = 0; k < 32; k += 2) {
-    const lds_t* cb = colv + 64 * ((k >> 1) & 1);
     lds_t* nb = colv + 64 * (((k >> 1) + 1) & 1);
-    // columns k and k+1, read while the pivots' reciprocal square roots are
-    // computed
-    T c0[32], c1[32];
-#pragma unroll
-    for (int q = k + 2; q < 32; ++q) {
-      c0[q] = cb[q];
-      c1[q] = cb[32 + q];
-    }
     const T a = bcast_lane(u[k], k);
     const T b = bcast_lane(u[k], k + 1);
     const T c = bcast_lane(u[k + 1], k + 1);
@@ -191,18 +194,17 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
     const T g = rsqrt_t(a * c - b * b);
     const T inv2 = g * (a * inv1);  // 1 / sqrt(c - b^2 / a)
     const T t = b * inv1 * inv1;
-    // step k: row lanes l = L[r][k] (0 for r <= k); inverse lanes z_k
-    const T keep1 = (rowlane && r <= k) ? T(0) : T(1);
-    const T lk1 = u[k] * inv1 * keep1;
+    // step k: row lanes l = L[r][k]; inverse lanes z_k
+    const T lk1 = u[k] * inv1;
     const T cf1 = lk1 * inv1;
     // step k+1, on entry k+1 as updated by step k (A[k+1][k] = b)
     const T uk1 = u[k + 1] - cf1 * b;
-    const T keep2 = (rowlane && r <= k + 1) ? T(0) : T(1);
-    const T lk2 = uk1 * inv2 * keep2;
+    const T lk2 = uk1 * inv2;
     const T cf2 = lk2 * inv2;
     const T cf1p = cf1 - cf2 * t;
     // (explicit fused multiply-adds: two per entry and pair of steps)
     T ncf1 = -cf1p, ncf2 = -cf2;
+    T n0[32], n1[32];
     if (k + 2 < 32) {
       u[k + 2] = fma(ncf2, c1[k + 2], fma(ncf1, c0[k + 2], u[k + 2]));
       u[k + 3] = fma(ncf2, c1[k + 3], fma(ncf1, c0[k + 3], u[k + 3]));
@@ -214,18 +216,34 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
       // the remaining updates use the coefficients as pinned here, after the
       // stores
       asm volatile("" : "+v"(ncf1), "+v"(ncf2) : : "memory");
+#pragma unroll
+      for (int q = k + 4; q < 32; ++q) {
+        n0[q] = nb[q];
+        n1[q] = nb[32 + q];
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
     for (int q = k + 4; q < 32; ++q) u[q] = fma(ncf2, c1[q], fma(ncf1, c0[q], u[q]));
     const T pk1 = a * inv1;                // sqrt(a)
     const T pk2 = (c - b * t) * inv2;      // sqrt(c - b^2 / a)
-    u[k] = rowlane ? ((r == k) ? pk1 : ((r > k) ? lk1 : u[k])) : lk1;
-    u[k + 1] = rowlane ? ((r == k + 1) ? pk2 : ((r > k + 1) ? lk2 : uk1)) : lk2;
-    rinv = (r == k) ? inv1 : ((r == k + 1) ? inv2 : rinv);
+    // lane comparisons on a copy of r the compiler cannot see through, so
+    // that they are made here and not hoisted out of the loop (32 hoisted
+    // masks spill to lanes of a vector register)
+    int rk = r;
+    asm volatile("" : "+v"(rk));
+    u[k] = rowlane ? ((rk == k) ? pk1 : ((rk > k) ? lk1 : u[k])) : lk1;
+    u[k + 1] = rowlane ? ((rk == k + 1) ? pk2 : ((rk > k + 1) ? lk2 : uk1)) : lk2;
+    rinv = (rk == k) ? inv1 : ((rk == k + 1) ? inv2 : rinv);
     // Materialise the registers every step: otherwise the optimiser sinks
     // the updates to their last use and keeps every broadcast value alive.
 #pragma unroll
     for (int q = 0; q < 32; ++q) asm volatile("" : "+v"(u[q]));
+#pragma unroll
+    for (int q = k + 4; q < 32; ++q) {
+      c0[q] = n0[q];
+      c1[q] = n1[q];
+    }
   }
 #pragma unroll
   for (int c = 0; c < 32; ++c)
