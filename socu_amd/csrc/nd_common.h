@@ -70,7 +70,20 @@ __host__ inline Level level_info(int N, int l) {
 // the same instruction (identical results for normal arguments); a pivot
 // below 2^-126 means a numerically singular block anyway.
 __device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ double rsqrt_t(double x) { return rsqrt(x); }
+// v_rsq_f64 (an estimate) refined by two Newton-Raphson steps,
+// y <- y + y (1 - x y^2) / 2, each of which doubles the correct bits.  The
+// pivots here are positive and normal; rsqrt(double) spends six selects per
+// call on zero, infinite and denormal arguments, on the pivot chain.  A
+// non-positive or NaN pivot still gives NaN (zero: inf * 0).
+__device__ __forceinline__ double rsqrt_t(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
 __device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
 __device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
 
