@@ -37,12 +37,19 @@ namespace socu {
 // general (padded) version.
 // TS is the storage type of L; T the compute type (TS = float, T = double
 // factors FP32 blocks with FP64 pivots and updates).
+// One 64-lane wavefront for NB == 1 (the whole factor is wave 0's pivot
+// chain: a workgroup of four would hold registers for three idle waves, and
+// batches of many small blocks are throughput-bound); four otherwise.
+constexpr int potrf_threads(int nb) { return nb == 1 ? 64 : kThreads; }
+
 template <typename TS, int NB, int NF, typename T = TS>
 __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, const uint8_t* __restrict__ mask, int N,
                                                        int n_, int s) {
   const int n = NF ? NF : n_;
   constexpr int NP = 32 * NB;
   constexpr int P = NP + 1;
+  constexpr int TH = potrf_threads(NB);
+  constexpr int WS = TH / 64 > 1 ? TH / 64 - 1 : 1;  // stride over waves 1.. (none when TH == 64)
   const int b = blockIdx.y;
   if (masked_out(mask, b)) return;
   const int i = s - 1 + 2 * s * blockIdx.x;
@@ -99,21 +106,21 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
   } else {
     // The block is read as one contiguous n*n run (coalesced, unconditional
     // loads from clamped addresses) and scattered into the padded LDS square.
-    constexpr int PER = NP * NP / kThreads;
+    constexpr int PER = NP * NP / TH;
     T v[PER];
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      const int idx = tid + e * kThreads;
+      const int idx = tid + e * TH;
       v[e] = (T)A[min(idx, nn - 1)];
     }
-    for (int idx = tid; idx < NP * NP; idx += kThreads) {
+    for (int idx = tid; idx < NP * NP; idx += TH) {
       const int r = idx / NP, c = idx - r * NP;
       S[r * P + c] = (r == c) ? T(1) : T(0);
     }
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      const int idx = tid + e * kThreads;
+      const int idx = tid + e * TH;
       const int r = idx / n, c = idx - r * n;
       if (idx < nn) S[r * P + c] = (c <= r) ? v[e] : T(0);
     }
@@ -187,14 +194,14 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
       if (kb > 0) trail(kb - 1, kb, kb);
       tile_chol_inv<T>(D, P, dinv + 32 * kb, col);
     } else {
-      for (int j = wave - 1; j < NB - 1; j += kThreads / 64 - 1) {
+      for (int j = wave - 1; j < NB - 1; j += WS) {
         if (j < kb - 1) inv_finish(kb - 1, j);
         if (j <= kb - 1) inv_partial(kb, j);
       }
       if (kb > 0) {
         const int m = NB - kb;
         const int cnt = m * (m + 1) / 2;  // triangle index 0 is (kb, kb)
-        for (int t = wave; t < cnt; t += kThreads / 64 - 1) {
+        for (int t = wave; t < cnt; t += WS) {
           int ti = 0, q = t;
           while (q > ti) { q -= ti + 1; ++ti; }
           trail(kb - 1, kb + ti, kb + q);
@@ -207,7 +214,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
         // its pivot chain.
         if (kb > 0) {
           const int nlow = NB - (kb - 1), nup = kb >= 2 ? kb - 2 : 0;
-          for (int t = wave - 1; t < nlow + nup; t += kThreads / 64 - 1) {
+          for (int t = wave - 1; t < nlow + nup; t += WS) {
             if (t < nlow) store_tile(kb - 1 + t, kb - 1);
             else store_tile(t - nlow, kb - 2);
           }
@@ -216,7 +223,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
     }
     __syncthreads();
     // panel: tile (ti, kb) <- tile (ti, kb) Linv^T
-    for (int ti = kb + 1 + wave; ti < NB; ti += kThreads / 64) {
+    for (int ti = kb + 1 + wave; ti < NB; ti += TH / 64) {
       T* X = S + 32 * ti * P + 32 * kb;
       typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
       const int li = l & 15, kq = l >> 4;
@@ -241,7 +248,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
   // The last row of inverse blocks (the others were formed during the
   // pivot chains above).
   if (wave > 0)
-    for (int j = wave - 1; j < NB - 1; j += kThreads / 64 - 1) inv_finish(NB - 1, j);
+    for (int j = wave - 1; j < NB - 1; j += WS) inv_finish(NB - 1, j);
   __syncthreads();
 
   // Write-back: tril(L) is the factor; the strict upper triangle, which is
@@ -251,14 +258,14 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
     // what the pivot steps have not written yet: diagonal tile NB-1 and the
     // inverse blocks Linv_{NB-2, j}^T, Linv_{NB-1, j}^T
     const int n2 = NB >= 2 ? NB - 2 : 0, cnt = 1 + n2 + (NB - 1);
-    for (int t = wave; t < cnt; t += kThreads / 64) {
+    for (int t = wave; t < cnt; t += TH / 64) {
       if (t == 0) store_tile(NB - 1, NB - 1);
       else if (t - 1 < n2) store_tile(t - 1, NB - 2);
       else store_tile(t - 1 - n2, NB - 1);
     }
   } else {
     // one contiguous run
-    for (int idx = tid; idx < nn; idx += kThreads) {
+    for (int idx = tid; idx < nn; idx += TH) {
       const int r = idx / n, c = idx - r * n;
       A[idx] = (TS)S[r * P + c];
     }
@@ -1205,11 +1212,11 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
   kt_begin(kKcPotrf, st);
   if (f64_pivots && sizeof(T) == 4) {
     const size_t lds_potrf = potrf_blk_lds<double, NB>();
-    hipLaunchKernelGGL((k_potrf_blk<T, NB, NF, double>), dim3(lv.dim, B), dim3(kThreads), lds_potrf, st, L, mask, N,
+    hipLaunchKernelGGL((k_potrf_blk<T, NB, NF, double>), dim3(lv.dim, B), dim3(potrf_threads(NB)), lds_potrf, st, L, mask, N,
                        n, lv.s);
   } else {
     const size_t lds_potrf = potrf_blk_lds<T, NB>();
-    hipLaunchKernelGGL((k_potrf_blk<T, NB, NF, T>), dim3(lv.dim, B), dim3(kThreads), lds_potrf, st, L, mask, N, n,
+    hipLaunchKernelGGL((k_potrf_blk<T, NB, NF, T>), dim3(lv.dim, B), dim3(potrf_threads(NB)), lds_potrf, st, L, mask, N, n,
                        lv.s);
   }
   kt_end(kKcPotrf, st);

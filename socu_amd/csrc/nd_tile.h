@@ -150,9 +150,11 @@ __device__ __forceinline__ void acc16_foreach(typename Mfma<T>::acc_t& acc, F f)
 // k+1 updated by step k).  Columns k, k+1 come from the LDS buffer (k/2) % 2
 // (LDS operations of one wavefront complete in issue order, so no barrier is
 // needed).  Look-ahead: entries k+2, k+3 are updated first and published as
-// the next pair's columns, and the next pair's column reads are issued right
-// after (a scheduling barrier keeps them there), so that their LDS latency
-// runs under this pair's remaining updates instead of the next pivot chain.
+// the next pair's columns.  FP32 (EARLY): the next pair's column reads are
+// issued right after (a scheduling barrier keeps them there), so that their
+// LDS latency runs under this pair's remaining updates instead of the next
+// pivot chain; FP64 reads them at the start of the pair step (two sets of FP64
+// columns in registers spill).
 // The coefficients are not masked for row lanes at or above the pair: those
 // lanes only change entries of their strict upper triangle, which nothing
 // reads (not the column buffers, the pivots, nor the store), and their own
@@ -178,15 +180,26 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
     colv[r] = u[0];
     colv[32 + r] = u[1];
   }
+  constexpr bool EARLY = sizeof(T) == 4;
   T c0[32], c1[32];
+  if constexpr (EARLY) {
 #pragma unroll
-  for (int q = 2; q < 32; ++q) {
-    c0[q] = colv[q];
-    c1[q] = colv[32 + q];
+    for (int q = 2; q < 32; ++q) {
+      c0[q] = colv[q];
+      c1[q] = colv[32 + q];
+    }
   }
 #pragma unroll
   for (int k = 0; k < 32; k += 2) {
     lds_t* nb = colv + 64 * (((k >> 1) + 1) & 1);
+    if constexpr (!EARLY) {
+      const lds_t* cb = colv + 64 * ((k >> 1) & 1);
+#pragma unroll
+      for (int q = k + 2; q < 32; ++q) {
+        c0[q] = cb[q];
+        c1[q] = cb[32 + q];
+      }
+    }
     const T a = bcast_lane(u[k], k);
     const T b = bcast_lane(u[k], k + 1);
     const T c = bcast_lane(u[k + 1], k + 1);
@@ -216,12 +229,14 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
       // the remaining updates use the coefficients as pinned here, after the
       // stores
       asm volatile("" : "+v"(ncf1), "+v"(ncf2) : : "memory");
+      if constexpr (EARLY) {
 #pragma unroll
-      for (int q = k + 4; q < 32; ++q) {
-        n0[q] = nb[q];
-        n1[q] = nb[32 + q];
+        for (int q = k + 4; q < 32; ++q) {
+          n0[q] = nb[q];
+          n1[q] = nb[32 + q];
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
     for (int q = k + 4; q < 32; ++q) u[q] = fma(ncf2, c1[q], fma(ncf1, c0[q], u[q]));
@@ -239,10 +254,12 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
     // the updates to their last use and keeps every broadcast value alive.
 #pragma unroll
     for (int q = 0; q < 32; ++q) asm volatile("" : "+v"(u[q]));
+    if constexpr (EARLY) {
 #pragma unroll
-    for (int q = k + 4; q < 32; ++q) {
-      c0[q] = n0[q];
-      c1[q] = n1[q];
+      for (int q = k + 4; q < 32; ++q) {
+        c0[q] = n0[q];
+        c1[q] = n1[q];
+      }
     }
   }
 #pragma unroll
