@@ -1155,8 +1155,11 @@ static hipError_t set_schur_small_attrs() {
 // which only costs where the level streams many blocks; measured on MI355X
 // (N=512, 1024) the small kernel wins up to about 12 MiB of the level's
 // blocks: n=128 FP64 factor 1207 -> 1093 us, FP32 875 -> 767 us, n=32 FP64
-// 265 -> 197 us.  SOCU_SCHUR_SMALL=<nodes> replaces the rule by a node count
-// (0 disables the small kernel).
+// 265 -> 197 us.  For n <= 32 it is taken at every level: k_schur_blk's
+// 64x64 output tiles leave three of four quarters outside a 32x32 block
+// (B=256 n=32 N=128 m=4 FP64 factor_and_solve: 2.36 -> 1.65 ms).
+// SOCU_SCHUR_SMALL=<nodes> replaces the rule by a node count (0 disables the
+// small kernel).
 static bool use_schur_small(int nodes, int n, int esize) {
   static int env = -2;
   if (env == -2) {
@@ -1164,6 +1167,7 @@ static bool use_schur_small(int nodes, int n, int esize) {
     env = e ? (atoi(e) < 0 ? 0 : atoi(e)) : -1;
   }
   if (env >= 0) return nodes <= env;
+  if (n <= 32) return true;
   return (double)nodes * n * n * esize <= 12.0 * 1024 * 1024;
 }
 
