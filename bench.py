@@ -32,8 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE, fetch scaled by the write/fetch ratio of the
 # bench's own device copies (write exact); per-launch numbers and the commands
 # in profiles/potrf_traffic_n128_N512_pmc.md.
-MEASURED_TRAFFIC = {("potrf", 128, 512, 1, "float32_f32_pivots"): 69122 * 1024,
-                    ("potrf", 128, 512, 1, "float32"): 69681 * 1024}
+MEASURED_TRAFFIC = {("potrf", 128, 512, 1, "float32_f32_pivots"): 56884 * 1024,
+                    ("potrf", 128, 512, 1, "float32"): 57811 * 1024}
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
