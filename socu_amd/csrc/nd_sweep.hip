@@ -570,7 +570,11 @@ hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, i
                             int* ws, T* slots, hipStream_t st) {
   if (B <= 0 || (!fwd && !bwd)) return hipSuccess;
   const int Lv = recursive_iterations(N);
-  const int lmin = sweep_lmin(B, N, n, (int)sizeof(T), fwd && bwd);
+  // The split counts the tasks of both directions even for a one-direction
+  // sweep: at n=128 N=512 the backward sweep after a factor with level 1
+  // persistent (its tasks alone fit) was 17 us slower than with level 1 as
+  // level kernels.
+  const int lmin = sweep_lmin(B, N, n, (int)sizeof(T), true);
   hipError_t e = hipSuccess;
   // forward below lmin: applies of levels 0..lmin-1 and their updates into
   // every receiving block (the last one without the next level's apply)
