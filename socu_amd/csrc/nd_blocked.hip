@@ -1207,7 +1207,8 @@ hipError_t init_blocked_kernels() {
 
 template <typename T, int NB, int NF>
 static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, bool f64_pivots,
-                                  hipStream_t st, hipEvent_t after_potrf, hipEvent_t after_trsm) {
+                                  hipStream_t st, hipEvent_t after_potrf, hipEvent_t after_trsm,
+                                  hipEvent_t before_trsm) {
   const int nE = (int)off_diag_storage_len(N);
   const int Lv = recursive_iterations(N);
   const Level lv = level_info(N, lidx);
@@ -1229,6 +1230,10 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
     if (e != hipSuccess) return e;
   }
   if (lidx == Lv - 1) return hipGetLastError();
+  if (before_trsm != nullptr) {
+    const hipError_t e = hipStreamWaitEvent(st, before_trsm, 0);
+    if (e != hipSuccess) return e;
+  }
   kt_begin(kKcTrsm, st);
   if (B * lv.dim >= trsm_node_min(NB)) {
     const int cpb = trsm_chunks_per_block();
@@ -1269,20 +1274,21 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
 
 template <typename T>
 hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, bool f64_pivots,
-                                hipStream_t st, hipEvent_t after_potrf, hipEvent_t after_trsm) {
+                                hipStream_t st, hipEvent_t after_potrf, hipEvent_t after_trsm,
+                                hipEvent_t before_trsm) {
   const bool fp = f64_pivots;
   switch (n) {
-    case 32: return factor_level_nb<T, 1, 32>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
-    case 64: return factor_level_nb<T, 2, 64>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
-    case 96: return factor_level_nb<T, 3, 96>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
-    case 128: return factor_level_nb<T, 4, 128>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
+    case 32: return factor_level_nb<T, 1, 32>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
+    case 64: return factor_level_nb<T, 2, 64>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
+    case 96: return factor_level_nb<T, 3, 96>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
+    case 128: return factor_level_nb<T, 4, 128>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
     default: break;
   }
   switch ((n + 31) / 32) {
-    case 1: return factor_level_nb<T, 1, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
-    case 2: return factor_level_nb<T, 2, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
-    case 3: return factor_level_nb<T, 3, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
-    case 4: return factor_level_nb<T, 4, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm);
+    case 1: return factor_level_nb<T, 1, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
+    case 2: return factor_level_nb<T, 2, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
+    case 3: return factor_level_nb<T, 3, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
+    case 4: return factor_level_nb<T, 4, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1387,6 +1393,7 @@ hipError_t overlap_create(SweepOverlap& ov) {
   hipError_t e = hipStreamCreateWithFlags(&ov.side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ov.fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ov.join, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ov.ready, hipEventDisableTiming);
   for (int l = 0; l < kSweepMaxLevels && e == hipSuccess; ++l) {
     e = hipEventCreateWithFlags(&ov.potrf[l], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ov.trsm[l], hipEventDisableTiming);
@@ -1402,6 +1409,7 @@ void overlap_destroy(SweepOverlap& ov) {
   }
   if (ov.fork) (void)hipEventDestroy(ov.fork);
   if (ov.join) (void)hipEventDestroy(ov.join);
+  if (ov.ready) (void)hipEventDestroy(ov.ready);
   if (ov.side) (void)hipStreamDestroy(ov.side);
   ov = SweepOverlap();
 }
@@ -1414,14 +1422,17 @@ void overlap_destroy(SweepOverlap& ov) {
 // does not touch.
 template <typename T>
 hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, const uint8_t* sweep_mask, int B, int N,
-                               int n, int m, bool f64_pivots, const SweepOverlap& ov, hipStream_t st) {
+                               int n, int m, bool f64_pivots, const SweepOverlap& ov, hipStream_t st,
+                               const std::function<hipError_t(hipStream_t)>* side_first) {
   const int Lv = recursive_iterations(N);
   hipError_t e = hipEventRecord(ov.fork, st);
   if (e == hipSuccess) e = hipStreamWaitEvent(ov.side, ov.fork, 0);
+  if (e == hipSuccess && side_first != nullptr) e = (*side_first)(ov.side);
+  if (e == hipSuccess && side_first != nullptr) e = hipEventRecord(ov.ready, ov.side);
   for (int l = 0; l < Lv && e == hipSuccess; ++l) {
     const bool last = l == Lv - 1;
     e = factor_level_blocked<T>(L, E, factor_mask, B, N, n, l, f64_pivots, st, ov.potrf[l],
-                                last ? nullptr : ov.trsm[l]);
+                                last ? nullptr : ov.trsm[l], (l == 0 && side_first != nullptr) ? ov.ready : nullptr);
     if (e == hipSuccess) e = hipStreamWaitEvent(ov.side, ov.potrf[l], 0);
     if (e == hipSuccess) e = forward_apply_blocked<T>(L, E, x, sweep_mask, B, N, n, m, l, ov.side);
     if (e == hipSuccess && !last) e = hipStreamWaitEvent(ov.side, ov.trsm[l], 0);
@@ -1433,14 +1444,16 @@ hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, con
 }
 
 template hipError_t factor_with_forward<float>(float*, float*, float*, const uint8_t*, const uint8_t*, int, int, int,
-                                               int, bool, const SweepOverlap&, hipStream_t);
+                                               int, bool, const SweepOverlap&, hipStream_t,
+                                               const std::function<hipError_t(hipStream_t)>*);
 template hipError_t factor_with_forward<double>(double*, double*, double*, const uint8_t*, const uint8_t*, int, int,
-                                                int, int, bool, const SweepOverlap&, hipStream_t);
+                                                int, int, bool, const SweepOverlap&, hipStream_t,
+                                                const std::function<hipError_t(hipStream_t)>*);
 
 template hipError_t factor_level_blocked<float>(float*, float*, const uint8_t*, int, int, int, int, bool,
-                                                hipStream_t, hipEvent_t, hipEvent_t);
+                                                hipStream_t, hipEvent_t, hipEvent_t, hipEvent_t);
 template hipError_t factor_level_blocked<double>(double*, double*, const uint8_t*, int, int, int, int, bool,
-                                                 hipStream_t, hipEvent_t, hipEvent_t);
+                                                 hipStream_t, hipEvent_t, hipEvent_t, hipEvent_t);
 template hipError_t forward_step_blocked<float>(const float*, const float*, float*, const uint8_t*, int, int, int, int,
                                                 int, hipStream_t, bool);
 template hipError_t forward_step_blocked<double>(const double*, const double*, double*, const uint8_t*, int, int, int,

@@ -1,6 +1,8 @@
 // Host-side launchers shared between the kernel files and the C-ABI.
 #pragma once
 
+#include <functional>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -34,10 +36,11 @@ hipError_t init_blocked_kernels();
 // inverted in FP64 and rounded to FP32 on store.
 // after_potrf / after_trsm (optional) are recorded on st after the level's
 // potrf and trsm launches (the forward sweep can follow them on another
-// stream).
+// stream); st waits for before_trsm (optional) between potrf and trsm.
 template <typename T>
 hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, bool f64_pivots,
-                                hipStream_t st, hipEvent_t after_potrf = nullptr, hipEvent_t after_trsm = nullptr);
+                                hipStream_t st, hipEvent_t after_potrf = nullptr, hipEvent_t after_trsm = nullptr,
+                                hipEvent_t before_trsm = nullptr);
 
 // Forward-substitution step: step -1 applies level 0; step l >= 0 applies
 // level l's updates to every receiving block, then (unless apply_next is
@@ -94,17 +97,20 @@ bool sweep_persistent_enabled(int n, int dsize);
 // destroyed with it).
 struct SweepOverlap {
   hipStream_t side = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, ready = nullptr;
   hipEvent_t potrf[kSweepMaxLevels] = {};
   hipEvent_t trsm[kSweepMaxLevels] = {};
 };
 hipError_t overlap_create(SweepOverlap& ov);
 void overlap_destroy(SweepOverlap& ov);
 // Factor levels on st with the forward sweep of x on ov.side, joined back
-// into st at the end (the backward sweep is the caller's).
+// into st at the end (the backward sweep is the caller's).  side_first
+// (optional) is enqueued on ov.side before the sweep; the first level's trsm
+// waits for it (it may prepare E, and anything the sweep reads).
 template <typename T>
 hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, const uint8_t* sweep_mask, int B, int N,
-                               int n, int m, bool f64_pivots, const SweepOverlap& ov, hipStream_t st);
+                               int n, int m, bool f64_pivots, const SweepOverlap& ov, hipStream_t st,
+                               const std::function<hipError_t(hipStream_t)>* side_first = nullptr);
 
 // Mixed-precision factor_and_solve with FP64 refinement (nd_refine.hip).
 struct MixedPlan {

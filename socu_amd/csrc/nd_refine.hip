@@ -507,39 +507,52 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
   const size_t xcount = (size_t)N * nm;
   hipError_t e = hipMemcpyAsync(p.b64, p.x, sizeof(double) * B * xcount, hipMemcpyDeviceToDevice, st);
   if (e != hipSuccess) return e;
-  // FP32 working copy of Psi; the fill-in region starts at zero.
+  // FP32 working copy of Psi (tril(L), E, fill-in region zeroed), in one or
+  // two launches: lcn elements of L, ecn of E, escn of E's whole storage.
   const size_t lc = (size_t)N * nn;
   const size_t ec = N > 1 ? (size_t)(N - 1) * nn : 0, es = N > 1 ? (size_t)nE * nn : 0;
-  {
-    const bool vec = lc % 4 == 0 && ec % 4 == 0 && es % 4 == 0 && ((uintptr_t)p.L % 16) == 0 &&
+  auto cast = [&](size_t lcn, size_t ecn, size_t escn, hipStream_t s) {
+    const bool vec = lcn % 4 == 0 && ecn % 4 == 0 && escn % 4 == 0 && ((uintptr_t)p.L % 16) == 0 &&
                      ((uintptr_t)p.E % 16) == 0 && ((uintptr_t)p.L32 % 16) == 0 && ((uintptr_t)p.E32 % 16) == 0;
-    const size_t units = (lc + es) / (vec ? 4 : 1);
+    const size_t units = (lcn + escn) / (vec ? 4 : 1);
     size_t blocks = (units + kThreads - 1) / kThreads;
     const size_t cap = (size_t)(2048 / B > 0 ? 2048 / B : 1);
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
-    const double* Ep = N > 1 ? p.E : p.L;  // unused when es == 0
+    const double* Ep = N > 1 ? p.E : p.L;  // unused when escn == 0
     float* E32p = N > 1 ? p.E32 : p.L32;
     if (vec)
-      hipLaunchKernelGGL((k_psi_to_f32<true>), dim3((unsigned)blocks, B), dim3(kThreads), 0, st, p.L32, E32p, p.L, Ep,
-                         lc, ec, es, n, p.mask);
+      hipLaunchKernelGGL((k_psi_to_f32<true>), dim3((unsigned)blocks, B), dim3(kThreads), 0, s, p.L32, E32p, p.L, Ep,
+                         lcn, ecn, escn, n, p.mask);
     else
-      hipLaunchKernelGGL((k_psi_to_f32<false>), dim3((unsigned)blocks, B), dim3(kThreads), 0, st, p.L32, E32p, p.L,
-                         Ep, lc, ec, es, n, p.mask);
-  }
-  const int Lv = recursive_iterations(N);
+      hipLaunchKernelGGL((k_psi_to_f32<false>), dim3((unsigned)blocks, B), dim3(kThreads), 0, s, p.L32, E32p, p.L,
+                         Ep, lcn, ecn, escn, n, p.mask);
+  };
   // x <- 0 and the first right-hand side d32 <- b: independent of the factor
-  hipLaunchKernelGGL(k_refine_init, dim3(p.nblk_init, B), dim3(kThreads), 0, st, p.x, p.b64, p.d32, p.partial, xcount,
-                     p.nblk_init, p.mask);
-  hipLaunchKernelGGL(k_refine_start, dim3(B), dim3(kThreads), 0, st, p.state, p.active, p.partial, p.nblk_init,
-                     p.mask);
+  auto init = [&](hipStream_t s) {
+    hipLaunchKernelGGL(k_refine_init, dim3(p.nblk_init, B), dim3(kThreads), 0, s, p.x, p.b64, p.d32, p.partial,
+                       xcount, p.nblk_init, p.mask);
+    hipLaunchKernelGGL(k_refine_start, dim3(B), dim3(kThreads), 0, s, p.state, p.active, p.partial, p.nblk_init,
+                       p.mask);
+  };
+  const int Lv = recursive_iterations(N);
   const bool overlap = p.overlap.side != nullptr;
   if (overlap) {
-    // the first correction solve's forward sweep runs alongside the factor
+    // The factor's first kernel needs only tril(L); E, the refinement state
+    // and the first correction solve's forward sweep go to the side stream,
+    // alongside the factor (the first trsm waits for E).
+    cast(lc, 0, 0, st);
+    const std::function<hipError_t(hipStream_t)> first = [&](hipStream_t s) {
+      cast(0, ec, es, s);
+      init(s);
+      return hipGetLastError();
+    };
     if ((e = factor_with_forward<float>(p.L32, p.E32, p.d32, p.mask, p.active, B, N, n, m, p.f64_pivots, p.overlap,
-                                        st)) != hipSuccess)
+                                        st, &first)) != hipSuccess)
       return e;
   } else {
+    cast(lc, ec, es, st);
+    init(st);
     for (int l = 0; l < Lv; ++l)
       if ((e = factor_level_blocked<float>(p.L32, p.E32, p.mask, B, N, n, l, p.f64_pivots, st)) != hipSuccess)
         return e;
