@@ -22,7 +22,10 @@ def _problem(n, N, m, alpha, seed, B=1):
 
 
 @pytest.mark.parametrize("n,N,m,alpha", [(32, 20, 1, 10.0), (64, 33, 4, 10.0), (128, 16, 1, 16.0), (36, 7, 3, 10.0),
-                                         (128, 64, 8, 16.0), (10, 100, 2, 10.0)])
+                                         (128, 64, 8, 16.0), (10, 100, 2, 10.0),
+                                         # one right-hand side (two-pass residual) at ragged n, one block, two blocks
+                                         (36, 9, 1, 10.0), (10, 33, 1, 10.0), (100, 5, 1, 16.0), (16, 1, 1, 10.0),
+                                         (24, 2, 1, 10.0)])
 @pytest.mark.parametrize("graph", [False, True])
 @pytest.mark.parametrize("precision", ["float32", "float32_f32_pivots"])
 def test_mixed_reaches_fp64_residual(n, N, m, alpha, graph, precision, device):
