@@ -82,10 +82,11 @@ hipError_t init_sweep_kernels();
 size_t sweep_ws_ints(int B, int N);
 size_t sweep_slot_elems(int B, int N, int n);
 // Forward and/or backward substitution of x (B, N, n, 1) with the factor in
-// L, E; one clear launch and one persistent launch.
+// L, E; one clear launch (skipped with clear = false, for callers that zero
+// the workspace themselves) and one persistent launch.
 template <typename T>
 hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, bool fwd, bool bwd,
-                            int* ws, T* slots, hipStream_t st);
+                            int* ws, T* slots, hipStream_t st, bool clear = true);
 // Whether a plan for block size n and element size dsize uses the persistent
 // sweeps: where they measured faster (FP32, n > 96) unless the environment
 // variable SOCU_SWEEP is "level" (never) or "persistent" (always).

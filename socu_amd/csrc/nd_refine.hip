@@ -114,11 +114,30 @@ __device__ __forceinline__ double block_sum(const double* __restrict__ part, int
   return red[0];
 }
 
+// Re-arms system b's part of the persistent sweep workspace for the next
+// sweep (the solve's sweeps then need no clearing launch): its per-node
+// counters, and from workgroup 0 the ticket and, with status, the status word
+// (zeroed once per solve, so that an expired wait stays visible to the host).
+__device__ __forceinline__ void sweep_ws_zero(int* __restrict__ ws, int B, int N, int b, bool status) {
+  if (ws == nullptr) return;
+  int* cnt = ws + kSweepHeader;
+  for (int k = threadIdx.x; k < 3 * N; k += kThreads) {
+    const int a = k / N;
+    cnt[(size_t)a * B * N + (size_t)b * N + (k - a * N)] = 0;
+  }
+  if (b == 0 && threadIdx.x == 0) {
+    ws[0] = 0;
+    if (status) ws[1] = 0;
+  }
+}
+
 // state[b] <- (||b||^2, ||b||^2, done = masked out, 0); one workgroup per b.
 __global__ __launch_bounds__(kThreads) void k_refine_start(double* __restrict__ state, uint8_t* __restrict__ active,
                                                           const double* __restrict__ partial, int nblk,
-                                                          const uint8_t* __restrict__ mask) {
+                                                          const uint8_t* __restrict__ mask, int* __restrict__ sweep_ws,
+                                                          int N) {
   const int b = blockIdx.x;
+  sweep_ws_zero(sweep_ws, gridDim.x, N, b, true);
   __shared__ double red[kThreads];
   const double s = block_sum(partial + (size_t)b * nblk, nblk, red);
   if (threadIdx.x == 0) {
@@ -287,8 +306,10 @@ __global__ __launch_bounds__(kThreads) void k_residual(const double* __restrict_
 // ||r||^2 from the partials; sets the done flag when ||r|| <= tol ||b||.
 // One workgroup per system.
 __global__ __launch_bounds__(kThreads) void k_refine_check(double* __restrict__ state, uint8_t* __restrict__ active,
-                                                          const double* __restrict__ partial, int nblk, double tol2) {
+                                                          const double* __restrict__ partial, int nblk, double tol2,
+                                                          int* __restrict__ sweep_ws, int N) {
   const int b = blockIdx.x;
+  sweep_ws_zero(sweep_ws, gridDim.x, N, b, false);
   if (state[4 * b + 2] != 0.0) return;
   __shared__ double red[kThreads];
   const double s = block_sum(partial + (size_t)b * nblk, nblk, red);
@@ -533,7 +554,7 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
     hipLaunchKernelGGL(k_refine_init, dim3(p.nblk_init, B), dim3(kThreads), 0, s, p.x, p.b64, p.d32, p.partial,
                        xcount, p.nblk_init, p.mask);
     hipLaunchKernelGGL(k_refine_start, dim3(B), dim3(kThreads), 0, s, p.state, p.active, p.partial, p.nblk_init,
-                       p.mask);
+                       p.mask, p.sweep_ws, N);
   };
   const int Lv = recursive_iterations(N);
   const bool overlap = p.overlap.side != nullptr;
@@ -563,8 +584,9 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
     if (p.sweep_ws != nullptr) {
       // the upper levels of both sweeps in one persistent launch; converged
       // systems are skipped
+      // (the workspace is re-armed by k_refine_start / k_refine_check)
       if ((e = sweep_persistent<float>(p.L32, p.E32, p.d32, p.active, B, N, n, fwd, true, p.sweep_ws, p.sweep_slots,
-                                       st)) != hipSuccess)
+                                       st, false)) != hipSuccess)
         return e;
     } else {
       if (fwd)
@@ -584,7 +606,7 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
       hipLaunchKernelGGL(k_resid1_finish, dim3(nb2, B), dim3(kThreads), 0, st, p.x, p.d32, p.res_w, p.res_v,
                          p.res_c, p.partial, p.state, N, n, nsl);
       hipLaunchKernelGGL(k_refine_check, dim3(B), dim3(kThreads), 0, st, p.state, p.active, p.partial, nb2,
-                         p.tol * p.tol);
+                         p.tol * p.tol, p.sweep_ws, N);
       kt_end(kKcRefine, st);
       continue;
     }
@@ -594,7 +616,7 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
     hipLaunchKernelGGL(k_residual, dim3(N * nsl, B), dim3(kThreads), rlds, st, p.L, p.E, p.x, p.b64, p.d32,
                        p.partial, p.state, N, n, m, nE, nsl);
     hipLaunchKernelGGL(k_refine_check, dim3(B), dim3(kThreads), 0, st, p.state, p.active, p.partial, N * nsl,
-                       p.tol * p.tol);
+                       p.tol * p.tol, p.sweep_ws, N);
     kt_end(kKcRefine, st);
   }
   return hipGetLastError();

@@ -567,7 +567,7 @@ int sweep_lmin(int B, int N, int n, int dsize, bool both) {
 
 template <typename T>
 hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, bool fwd, bool bwd,
-                            int* ws, T* slots, hipStream_t st) {
+                            int* ws, T* slots, hipStream_t st, bool clear) {
   if (B <= 0 || (!fwd && !bwd)) return hipSuccess;
   const int Lv = recursive_iterations(N);
   // The split counts the tasks of both directions even for a one-direction
@@ -595,7 +595,8 @@ hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, i
   const int nclear = (int)sweep_ws_ints(B, N);
   const int cblocks = (nclear + kThreads - 1) / kThreads;
   kt_begin(fwd ? kKcForward : kKcBackward, st);
-  hipLaunchKernelGGL(k_sweep_clear, dim3(cblocks < 1024 ? cblocks : 1024), dim3(kThreads), 0, st, ws, nclear);
+  if (clear)
+    hipLaunchKernelGGL(k_sweep_clear, dim3(cblocks < 1024 ? cblocks : 1024), dim3(kThreads), 0, st, ws, nclear);
   switch (nb) {
     case 1: e = launch_sweep<T, 1>(L, E, x, mask, ws, slots, B, N, n, dirs, lmin, grid, st); break;
     case 2: e = launch_sweep<T, 2>(L, E, x, mask, ws, slots, B, N, n, dirs, lmin, grid, st); break;
@@ -612,8 +613,8 @@ hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, i
 }
 
 template hipError_t sweep_persistent<float>(const float*, const float*, float*, const uint8_t*, int, int, int, bool,
-                                            bool, int*, float*, hipStream_t);
+                                            bool, int*, float*, hipStream_t, bool);
 template hipError_t sweep_persistent<double>(const double*, const double*, double*, const uint8_t*, int, int, int,
-                                             bool, bool, int*, double*, hipStream_t);
+                                             bool, bool, int*, double*, hipStream_t, bool);
 
 }  // namespace socu
