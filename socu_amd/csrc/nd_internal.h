@@ -88,8 +88,10 @@ template <typename T>
 hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, bool fwd, bool bwd,
                             int* ws, T* slots, hipStream_t st, bool clear = true);
 // Whether a plan for block size n and element size dsize uses the persistent
-// sweeps: where they measured faster (FP32, n > 96) unless the environment
-// variable SOCU_SWEEP is "level" (never) or "persistent" (always).
+// sweeps (plans with one right-hand side only): yes, they measured faster than
+// the level kernels for every block size and precision, unless the
+// environment variable SOCU_SWEEP is "level" (never) or "persistent"
+// (always, also the default).
 bool sweep_persistent_enabled(int n, int dsize);
 
 // A side stream and per-level events that let the forward sweep of a

@@ -559,7 +559,9 @@ bool sweep_persistent_enabled(int n, int dsize) {
   const char* v = getenv("SOCU_SWEEP");
   if (v != nullptr && strcmp(v, "level") == 0) return false;
   if (v != nullptr && strcmp(v, "persistent") == 0) return true;
-  return dsize == 4 && n > 96;
+  (void)n;
+  (void)dsize;
+  return true;  // faster than the level kernels for every block size measured (see sweep_lmin)
 }
 
 size_t sweep_ws_ints(int B, int N) { return (size_t)kSweepHeader + 3 * (size_t)B * N; }
@@ -569,22 +571,20 @@ size_t sweep_slot_elems(int B, int N, int n) {
   return (size_t)B * (size_t)(k > 0 ? k : 1) * n;
 }
 
-// Lowest level swept by the persistent launch: the smallest lmin whose tasks
-// (levels lmin.. in each requested direction) all fit the resident
-// workgroups, so every task is claimed, and its blocks loaded, at the start.
-// The wide levels below it stream through the level kernels, whose
-// bandwidth does not depend on how many tasks are resident.
-int sweep_lmin(int B, int N, int n, int dsize, bool both) {
+// Lowest level swept by the persistent launch: levels pay there while the
+// widest of them has at most W nodes over the batch, with W = 64 for n <= 64,
+// 256 for FP32 and 1024 for FP64 blocks above that.  Measured solve times
+// (N=512, one right-hand side, lmin 0..5 against the level kernels) for
+// B = 1, 4 and n = 32, 64, 96, 128 in both precisions: this choice is the
+// fastest or within 4% of it in each case (FP64 n=128 B=1: 226 us against
+// 341 us with level kernels only, FP32: 127 us against 220 us).  Below lmin
+// the level kernels stream the wide levels.
+int sweep_lmin(int B, int N, int n, int dsize) {
   const int Lv = recursive_iterations(N);
-  const int nb = (n + 31) / 32;
-  const int resident = (g_cus > 0 ? g_cus : 1) * g_sweep_occ[dsize == 8 ? 1 : 0][nb - 1];
-  int tasks = 0, lmin = Lv;
-  for (int l = Lv - 1; l >= 0; --l) {
-    tasks += ((N + (1 << l)) >> (l + 1)) * B * (both ? 2 : 1);
-    if (tasks > resident) break;
-    lmin = l;
-  }
-  return lmin < Lv ? lmin : Lv - 1;
+  const long long W = n <= 64 ? 64 : (dsize == 8 ? 1024 : 256);
+  int lmin = 0;
+  while (lmin < Lv - 1 && (long long)B * ((N + (1 << lmin)) >> (lmin + 1)) > W) ++lmin;
+  return lmin;
 }
 
 template <typename T>
@@ -592,11 +592,7 @@ hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, i
                             int* ws, T* slots, hipStream_t st, bool clear) {
   if (B <= 0 || (!fwd && !bwd)) return hipSuccess;
   const int Lv = recursive_iterations(N);
-  // The split counts the tasks of both directions even for a one-direction
-  // sweep: at n=128 N=512 the backward sweep after a factor with level 1
-  // persistent (its tasks alone fit) was 17 us slower than with level 1 as
-  // level kernels.
-  const int lmin = sweep_lmin(B, N, n, (int)sizeof(T), true);
+  const int lmin = sweep_lmin(B, N, n, (int)sizeof(T));
   hipError_t e = hipSuccess;
   // forward below lmin: applies of levels 0..lmin-1 and their updates into
   // every receiving block (the last one without the next level's apply)

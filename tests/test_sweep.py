@@ -1,9 +1,9 @@
 """Persistent dataflow sweeps (one right-hand side) against the oracle.
 
-Plans with m == 1 can run the upper ND levels of forward and backward
-substitution as one persistent launch of node tasks (by default only for FP32
-blocks with n > 96; SOCU_SWEEP=persistent forces it, which every test here
-does unless it compares with the level kernels); this file checks that path block for block against
+Plans with m == 1 run the upper ND levels of forward and backward
+substitution as one persistent launch of node tasks (the default;
+SOCU_SWEEP=persistent forces it, which every test here does unless it
+compares with the level kernels); this file checks that path block for block against
 oracle.nd (forward result y and solution x; FP64 to 1e-11 relative, FP32 to
 2e-4), against the level-by-level sweep kernels (selected with
 SOCU_SWEEP=level), under masks, in graph replays, for run-to-run
@@ -214,14 +214,15 @@ def test_graph_replays_and_reproducibility(device):
 
 def test_default_selection(device):
     """Without an override the plans pick the persistent sweeps exactly for
-    FP32 blocks with n > 96 and one right-hand side."""
+    one right-hand side, in both precisions and at every block size."""
     import torch
     from socu_amd.block_tridiag_solver import create_cholesky_solve_launch
 
     with _SweepMode(None):
         for n, dt, m, want in [(128, torch.float32, 1, True), (97, torch.float32, 1, True),
-                               (96, torch.float32, 1, False), (128, torch.float64, 1, False),
-                               (128, torch.float32, 2, False)]:
+                               (96, torch.float32, 1, True), (128, torch.float64, 1, True),
+                               (8, torch.float64, 1, True), (128, torch.float32, 2, False),
+                               (32, torch.float64, 4, False)]:
             L = torch.zeros(1, 3, n, n, dtype=dt, device=device)
             E = torch.zeros(1, 2, n, n, dtype=dt, device=device)
             x = torch.zeros(1, 3, n, m, dtype=dt, device=device)
