@@ -342,7 +342,10 @@ int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const u
   socu_plan* p = new socu_plan{op, L, E, x, mask, B, N, n, need_x ? m : 1, dtype, nullptr, nullptr, SweepWs(),
                                socu::SweepOverlap(), false};
   rc = alloc_sweep_ws(p->sweep, op, B, N, n, p->m, dtype);
-  if (rc == SOCU_OK && op == SOCU_OP_FACTOR_AND_SOLVE && B > 0) {
+  // The forward sweep alongside the factor pays inside a graph only: issued
+  // eagerly, its event records and waits cost more host time than the
+  // overlap saves (n=128 N=512 FP64: 1.19 ms without, 1.24 ms with).
+  if (rc == SOCU_OK && op == SOCU_OP_FACTOR_AND_SOLVE && B > 0 && use_graph) {
     const hipError_t e = socu::overlap_create(p->overlap);
     if (e != hipSuccess) rc = hip_fail(e, "stream / event creation (factor_and_solve)");
     p->has_overlap = e == hipSuccess;
@@ -448,6 +451,7 @@ int socu_mixed_create(socu_mixed** out, const void* L, const void* E, void* x, c
   h->p.tol = tol;
   h->p.max_iter = max_iter;
   h->p.f64_pivots = precision == SOCU_PREC_FP32;
+  h->p.use_overlap = use_graph != 0;  // a second stream pays inside a graph only
   h->graph = nullptr;
   h->exec = nullptr;
   if (B == 0) {

@@ -505,7 +505,7 @@ hipError_t mixed_alloc(MixedPlan& p) {
     if ((e = hipMemset(p.sweep_ws, 0, sizeof(int) * sweep_ws_ints(p.B, p.N))) != hipSuccess) return e;
     if ((e = hipMalloc(&p.sweep_slots, sizeof(float) * sweep_slot_elems(p.B, p.N, p.n))) != hipSuccess) return e;
   }
-  return overlap_create(p.overlap);
+  return p.use_overlap ? overlap_create(p.overlap) : hipSuccess;
 }
 
 void mixed_free(MixedPlan& p) {
