@@ -225,8 +225,8 @@ def bench_adjoint(args, dev, st):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(independent systems shard over GPUs with no communication; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only; runs on several GPUs are not supported "
+              "(independent systems would shard over GPUs with no communication).", file=sys.stderr)
         sys.exit(2)
 
     import torch
