@@ -87,6 +87,25 @@ __device__ __forceinline__ double rsqrt_t(double x) {
 __device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
 __device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
 
+// Agent-scope relaxed atomic loads and stores (global_load/store ... sc1): the
+// L2 of an XCD writes such stores through and does not serve such loads
+// stale, so values handed between workgroups of one kernel need no L2
+// write-back or invalidation; stores_done() waits until this thread's stores
+// have completed (they then precede its later stores and counter updates).
+template <typename T>
+__device__ __forceinline__ T ld_coh(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_coh(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stores_done() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0);
+  asm volatile("" ::: "memory");
+}
+
 // True when batch entry b is switched off by the mask.
 __device__ __forceinline__ bool masked_out(const uint8_t* mask, int b) {
   return mask != nullptr && mask[b] == 0;

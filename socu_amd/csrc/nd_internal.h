@@ -139,6 +139,7 @@ struct MixedPlan {
   double* res_w = nullptr;     // one-RHS residual pieces (m == 1), else NULL
   double* res_v = nullptr;
   double* res_c = nullptr;
+  int* res_ticket = nullptr;   // per system: residual workgroups done
   bool use_overlap = false;     // set before mixed_alloc
   SweepOverlap overlap;        // first correction solve's forward sweep alongside the factor
 };

@@ -132,12 +132,14 @@ __device__ __forceinline__ void sweep_ws_zero(int* __restrict__ ws, int B, int N
 }
 
 // state[b] <- (||b||^2, ||b||^2, done = masked out, 0); one workgroup per b.
+// Also zeroes ticket[b] (the one-RHS residual's count of finished workgroups).
 __global__ __launch_bounds__(kThreads) void k_refine_start(double* __restrict__ state, uint8_t* __restrict__ active,
                                                           const double* __restrict__ partial, int nblk,
                                                           const uint8_t* __restrict__ mask, int* __restrict__ sweep_ws,
-                                                          int N) {
+                                                          int* __restrict__ ticket, int N) {
   const int b = blockIdx.x;
   sweep_ws_zero(sweep_ws, gridDim.x, N, b, true);
+  if (ticket != nullptr && threadIdx.x == 0) ticket[b] = 0;
   __shared__ double red[kThreads];
   const double s = block_sum(partial + (size_t)b * nblk, nblk, red);
   if (threadIdx.x == 0) {
@@ -333,10 +335,10 @@ __global__ __launch_bounds__(kThreads) void k_refine_check(double* __restrict__ 
 //     every column k, summed over the slice's rows:
 //                           C_{i,s}[k] = sum_{r>k} L_i[r][k] y_i[r] + sum_r E_i[r][k] y_{i+1}[r]
 //   k_resid1_finish: x += d, r_i = W_i - sum_s C_{i,s} - V_i, r32 <- r and the
-//     ||r||^2 partials (k_refine_check then sums them and applies the
-//     convergence test; a last-workgroup-done counter in place of that launch
-//     measured slower: its release fences write the L2 back in every
-//     workgroup).
+//     ||r||^2 partials; the system's last workgroup to finish (a relaxed
+//     counter; partials pass through agent-scope atomic stores and loads, so
+//     no L2 write-back fence is needed) sums them in a fixed order and
+//     applies the convergence test.
 // Lanes: 16 per row (each load instruction reads 128 contiguous bytes of four
 // rows), row r = 32 s + (tid >> 4) + 16 g, column k = (tid & 15) + 16 j.
 __global__ __launch_bounds__(kThreads) void k_resid1_blocks(const double* __restrict__ L, const double* __restrict__ E,
@@ -423,12 +425,25 @@ __global__ __launch_bounds__(kThreads) void k_resid1_blocks(const double* __rest
   }
 }
 
-// dr holds the correction d on entry and the FP32 residual on exit.
+// dr holds the correction d on entry and the FP32 residual on exit.  The
+// workgroups of all systems also re-arm the persistent sweep workspace, as
+// k_refine_check does.
 __global__ __launch_bounds__(kThreads) void k_resid1_finish(double* __restrict__ x, float* __restrict__ dr,
                                                            const double* __restrict__ W, const double* __restrict__ V,
                                                            const double* __restrict__ C, double* __restrict__ partial,
-                                                           const double* __restrict__ state, int N, int n, int nsl) {
+                                                           double* __restrict__ state, uint8_t* __restrict__ active,
+                                                           int* __restrict__ ticket, double tol2,
+                                                           int* __restrict__ sweep_ws, int N, int n, int nsl) {
   const int b = blockIdx.y;
+  if (sweep_ws != nullptr) {
+    int* cnt = sweep_ws + kSweepHeader;
+    const int B = gridDim.y;
+    for (int k = blockIdx.x * kThreads + threadIdx.x; k < 3 * N; k += gridDim.x * kThreads) {
+      const int a = k / N;
+      cnt[(size_t)a * B * N + (size_t)b * N + (k - a * N)] = 0;
+    }
+    if (b == 0 && blockIdx.x == 0 && threadIdx.x == 0) sweep_ws[0] = 0;
+  }
   if (refine_done(state, b)) return;
   const int count = N * n;  // < 2^31 (checked by the host)
   __shared__ double red[kThreads / 64];
@@ -449,7 +464,36 @@ __global__ __launch_bounds__(kThreads) void k_resid1_finish(double* __restrict__
   for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
   __syncthreads();
-  if (threadIdx.x == 0) partial[(size_t)b * gridDim.x + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  double* part = partial + (size_t)b * gridDim.x;
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+    st_coh(part + blockIdx.x, (red[0] + red[1]) + (red[2] + red[3]));
+    stores_done();
+    const int done = __hip_atomic_fetch_add(ticket + b, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    last = done == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  // the system's last workgroup: sum the partials as k_refine_check does
+  __shared__ double tot[kThreads];
+  double s = 0.0;
+  for (int q = threadIdx.x; q < (int)gridDim.x; q += kThreads) s += ld_coh(part + q);
+  tot[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) tot[threadIdx.x] += tot[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double s2 = tot[0];
+    ticket[b] = 0;
+    state[4 * b + 1] = s2;
+    state[4 * b + 3] += 1.0;
+    if (s2 <= tol2 * state[4 * b + 0] || !(s2 == s2)) {  // converged, or NaN (non-SPD input)
+      state[4 * b + 2] = 1.0;
+      active[b] = 0;
+    }
+  }
 }
 
 }  // namespace socu
@@ -499,6 +543,7 @@ hipError_t mixed_alloc(MixedPlan& p) {
     if ((e = hipMalloc(&p.res_w, sizeof(double) * rows)) != hipSuccess) return e;
     if ((e = hipMalloc(&p.res_v, sizeof(double) * rows)) != hipSuccess) return e;
     if ((e = hipMalloc(&p.res_c, sizeof(double) * rows * residual_slices(p.n))) != hipSuccess) return e;
+    if ((e = hipMalloc(&p.res_ticket, sizeof(int) * p.B)) != hipSuccess) return e;
   }
   if (p.m == 1 && sweep_persistent_enabled(p.n, (int)sizeof(float))) {
     if ((e = hipMalloc(&p.sweep_ws, sizeof(int) * sweep_ws_ints(p.B, p.N))) != hipSuccess) return e;
@@ -510,7 +555,7 @@ hipError_t mixed_alloc(MixedPlan& p) {
 
 void mixed_free(MixedPlan& p) {
   void* ptrs[] = {p.L32,    p.E32,         p.d32,   p.b64,   p.state, p.partial,  p.active,
-                  p.sweep_ws, p.sweep_slots, p.res_w, p.res_v, p.res_c};
+                  p.sweep_ws, p.sweep_slots, p.res_w, p.res_v, p.res_c, p.res_ticket};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   p.L32 = p.E32 = p.d32 = nullptr;
@@ -519,6 +564,7 @@ void mixed_free(MixedPlan& p) {
   p.sweep_ws = nullptr;
   p.sweep_slots = nullptr;
   p.res_w = p.res_v = p.res_c = nullptr;
+  p.res_ticket = nullptr;
   overlap_destroy(p.overlap);
 }
 
@@ -554,7 +600,7 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
     hipLaunchKernelGGL(k_refine_init, dim3(p.nblk_init, B), dim3(kThreads), 0, s, p.x, p.b64, p.d32, p.partial,
                        xcount, p.nblk_init, p.mask);
     hipLaunchKernelGGL(k_refine_start, dim3(B), dim3(kThreads), 0, s, p.state, p.active, p.partial, p.nblk_init,
-                       p.mask, p.sweep_ws, N);
+                       p.mask, p.sweep_ws, p.res_ticket, N);
   };
   const int Lv = recursive_iterations(N);
   const bool overlap = p.overlap.side != nullptr;
@@ -584,7 +630,8 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
     if (p.sweep_ws != nullptr) {
       // the upper levels of both sweeps in one persistent launch; converged
       // systems are skipped
-      // (the workspace is re-armed by k_refine_start / k_refine_check)
+      // (the workspace is re-armed by k_refine_start and by k_resid1_finish
+      // or k_refine_check)
       if ((e = sweep_persistent<float>(p.L32, p.E32, p.d32, p.active, B, N, n, fwd, true, p.sweep_ws, p.sweep_slots,
                                        st, false)) != hipSuccess)
         return e;
@@ -604,9 +651,7 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
                          p.res_w, p.res_v, p.res_c, p.state, N, n, nE, nsl);
       const int nb2 = refine_blocks(xcount);
       hipLaunchKernelGGL(k_resid1_finish, dim3(nb2, B), dim3(kThreads), 0, st, p.x, p.d32, p.res_w, p.res_v,
-                         p.res_c, p.partial, p.state, N, n, nsl);
-      hipLaunchKernelGGL(k_refine_check, dim3(B), dim3(kThreads), 0, st, p.state, p.active, p.partial, nb2,
-                         p.tol * p.tol, p.sweep_ws, N);
+                         p.res_c, p.partial, p.state, p.active, p.res_ticket, p.tol * p.tol, p.sweep_ws, N, n, nsl);
       kt_end(kKcRefine, st);
       continue;
     }

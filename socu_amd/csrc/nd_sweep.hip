@@ -206,27 +206,10 @@ __device__ __forceinline__ bool sweep_decode(int task, int B, int N, int Lv, int
 }
 
 // Values passed between tasks (contribution slots, y_j, x_j) are written and
-// read as agent-scope relaxed atomics: the L2 of an XCD writes them through
-// and does not serve them stale, so a producer only waits for its stores to
-// complete before raising a counter, and a consumer reads them after seeing
-// the counter, with no release fence (which writes back the whole L2: 1 us for
-// a lone task, 7-9 us per task with 32-64 tasks in flight) and no acquire
-// invalidation.
-template <typename T>
-__device__ __forceinline__ T ld_coh(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void st_coh(T* p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Every store of this thread complete (the coherent stores then precede any
-// later store or counter update in device order).
-__device__ __forceinline__ void stores_done() {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt(0);
-  asm volatile("" ::: "memory");
-}
+// read with ld_coh / st_coh (nd_common.h): a producer waits for its stores to
+// complete before raising a counter, a consumer reads after seeing it, with
+// no release fence (which writes back the whole L2: 1 us for a lone task,
+// 7-9 us per task with 32-64 tasks in flight) and no acquire invalidation.
 
 // Bounded wait (one thread): until *p >= target.  Returns false, and raises
 // the status word, when the bound expires or an earlier wait already expired.
