@@ -181,12 +181,23 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
     colv[32 + r] = u[1];
   }
   constexpr bool EARLY = sizeof(T) == 4;
+  // FP32 column reads are 16-byte vector reads of four entries from an
+  // aligned start (col is 16-byte aligned; entries below the first one
+  // needed are dropped): half the LDS instructions of paired 4-byte reads.
+  typedef T t4 __attribute__((ext_vector_type(4)));
+  typedef __attribute__((address_space(3))) t4 lds_t4;
   T c0[32], c1[32];
   if constexpr (EARLY) {
 #pragma unroll
-    for (int q = 2; q < 32; ++q) {
-      c0[q] = colv[q];
-      c1[q] = colv[32 + q];
+    for (int q0 = 0; q0 < 32; q0 += 4) {
+      const t4 v0 = *(const lds_t4*)(colv + q0);
+      const t4 v1 = *(const lds_t4*)(colv + 32 + q0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (q0 + j >= 2) {
+          c0[q0 + j] = v0[j];
+          c1[q0 + j] = v1[j];
+        }
     }
   }
 #pragma unroll
@@ -231,9 +242,15 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
       asm volatile("" : "+v"(ncf1), "+v"(ncf2) : : "memory");
       if constexpr (EARLY) {
 #pragma unroll
-        for (int q = k + 4; q < 32; ++q) {
-          n0[q] = nb[q];
-          n1[q] = nb[32 + q];
+        for (int q0 = (k + 4) & ~3; q0 < 32; q0 += 4) {
+          const t4 v0 = *(const lds_t4*)(nb + q0);
+          const t4 v1 = *(const lds_t4*)(nb + 32 + q0);
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (q0 + j >= k + 4) {
+              n0[q0 + j] = v0[j];
+              n1[q0 + j] = v1[j];
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
       }
