@@ -180,6 +180,15 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
     colv[r] = u[0];
     colv[32 + r] = u[1];
   }
+  // Publishing a pair's columns: row lanes write buffer p, inverse lanes the
+  // other buffer, whose columns are already in registers (so no branch on
+  // the lane kind): pubv[p] is each lane's target for buffer p.
+  lds_t* pubv0 = colv + (rowlane ? 0 : 64);
+  lds_t* pubv1 = colv + (rowlane ? 64 : 0);
+  asm volatile("" : "+v"(pubv0), "+v"(pubv1));
+  // Row index for the pivot selects below; inverse lanes take one past every
+  // pivot, for which the selects give them the multipliers (no branch).
+  const int rsel = rowlane ? r : 64;
   constexpr bool EARLY = sizeof(T) == 4;
   // FP32 column reads are 16-byte vector reads of four entries from an
   // aligned start (col is 16-byte aligned; entries below the first one
@@ -203,6 +212,7 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
 #pragma unroll
   for (int k = 0; k < 32; k += 2) {
     lds_t* nb = colv + 64 * (((k >> 1) + 1) & 1);
+    lds_t* pb = (((k >> 1) + 1) & 1) ? pubv1 : pubv0;
     if constexpr (!EARLY) {
       const lds_t* cb = colv + 64 * ((k >> 1) & 1);
 #pragma unroll
@@ -233,10 +243,8 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
       u[k + 2] = fma(ncf2, c1[k + 2], fma(ncf1, c0[k + 2], u[k + 2]));
       u[k + 3] = fma(ncf2, c1[k + 3], fma(ncf1, c0[k + 3], u[k + 3]));
       asm volatile("" : "+v"(u[k + 2]), "+v"(u[k + 3]));
-      if (rowlane) {
-        nb[r] = u[k + 2];
-        nb[32 + r] = u[k + 3];
-      }
+      pb[r] = u[k + 2];
+      pb[32 + r] = u[k + 3];
       // the remaining updates use the coefficients as pinned here, after the
       // stores
       asm volatile("" : "+v"(ncf1), "+v"(ncf2) : : "memory");
@@ -262,10 +270,10 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
     // lane comparisons on a copy of r the compiler cannot see through, so
     // that they are made here and not hoisted out of the loop (32 hoisted
     // masks spill to lanes of a vector register)
-    int rk = r;
+    int rk = rsel;
     asm volatile("" : "+v"(rk));
-    u[k] = rowlane ? ((rk == k) ? pk1 : ((rk > k) ? lk1 : u[k])) : lk1;
-    u[k + 1] = rowlane ? ((rk == k + 1) ? pk2 : ((rk > k + 1) ? lk2 : uk1)) : lk2;
+    u[k] = (rk == k) ? pk1 : ((rk > k) ? lk1 : u[k]);
+    u[k + 1] = (rk == k + 1) ? pk2 : ((rk > k + 1) ? lk2 : uk1);
     rinv = (rk == k) ? inv1 : ((rk == k + 1) ? inv2 : rinv);
     // Materialise the registers every step: otherwise the optimiser sinks
     // the updates to their last use and keeps every broadcast value alive.
