@@ -187,7 +187,7 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
   lds_t* pubv1 = colv + (rowlane ? 64 : 0);
   asm volatile("" : "+v"(pubv0), "+v"(pubv1));
   // Row index for the pivot selects below; inverse lanes take one past every
-  // pivot, for which the selects give them the multipliers (no branch).
+  // pivot, so that the selects give them the multipliers (no branch).
   const int rsel = rowlane ? r : 64;
   constexpr bool EARLY = sizeof(T) == 4;
   // FP32 column reads are 16-byte vector reads of four entries from an
@@ -270,11 +270,14 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
     // lane comparisons on a copy of r the compiler cannot see through, so
     // that they are made here and not hoisted out of the loop (32 hoisted
     // masks spill to lanes of a vector register)
+    // (rows above k and k+1 take the multipliers too: for them these are
+    // strict-upper entries, which nothing reads)
     int rk = rsel;
     asm volatile("" : "+v"(rk));
-    u[k] = (rk == k) ? pk1 : ((rk > k) ? lk1 : u[k]);
-    u[k + 1] = (rk == k + 1) ? pk2 : ((rk > k + 1) ? lk2 : uk1);
-    rinv = (rk == k) ? inv1 : ((rk == k + 1) ? inv2 : rinv);
+    const bool on1 = rk == k, on2 = rk == k + 1;
+    u[k] = on1 ? pk1 : lk1;
+    u[k + 1] = on2 ? pk2 : lk2;
+    rinv = on1 ? inv1 : (on2 ? inv2 : rinv);
     // Materialise the registers every step: otherwise the optimiser sinks
     // the updates to their last use and keeps every broadcast value alive.
 #pragma unroll
