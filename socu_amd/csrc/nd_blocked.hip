@@ -280,8 +280,13 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
 // Both are MFMA products with the upper tiles (kb, q), kb <= q, of U; no
 // sequential substitution remains.
 // ---------------------------------------------------------------------------
+// Eight wavefronts: output tile q of the chunk is summed over kb <= q by two
+// of them (kb below and from h(q) = (q + 1) / 2), which meet in LDS, so the
+// longest product chain is half as long as with one wavefront per tile.
+constexpr int kTrsmThreads = 512;
+
 template <typename T, int NB, int NF>
-__global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, T* __restrict__ E,
+__global__ __launch_bounds__(kTrsmThreads) void k_trsm_blk(const T* __restrict__ L, T* __restrict__ E,
                                                       const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
                                                       int off) {
   const int n = NF ? NF : n_;
@@ -312,7 +317,8 @@ __global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, 
   // (All loads are unconditional from clamped addresses and pinned only
   // after the last one is issued, so they are in flight together; the
   // masking and reciprocals follow.)
-  constexpr int PT = 32 * 32 / kThreads;
+  constexpr int TT = kTrsmThreads;
+  constexpr int PT = 32 * 32 / TT;
   T uv[NT][PT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
@@ -320,16 +326,16 @@ __global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, 
     while (kb > q) { kb -= q + 1; ++q; }  // t = q(q+1)/2 + kb, kb <= q
 #pragma unroll
     for (int e = 0; e < PT; ++e) {
-      const int idx = tid + e * kThreads;
+      const int idx = tid + e * TT;
       const int R = 32 * kb + (idx >> 5), C = 32 * q + (idx & 31);
       uv[t][e] = A[(size_t)min(R, n - 1) * n + min(C, n - 1)];
     }
   }
-  constexpr int PZ = 32 * NP / kThreads;
+  constexpr int PZ = 32 * NP / TT;
   T zv[PZ];
 #pragma unroll
   for (int e = 0; e < PZ; ++e) {
-    const int idx = tid + e * kThreads;
+    const int idx = tid + e * TT;
     const int rr = idx / NP, kr = idx - rr * NP;  // R[c0+rr][kr], row reads
     const int kc = idx >> 5, cc = idx & 31;       // Lc[kc][c0+cc]
     const size_t ar = (size_t)(c0 + min(rr, w - 1)) * n + min(kr, n - 1);
@@ -348,7 +354,7 @@ __global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, 
     while (kb > q) { kb -= q + 1; ++q; }
 #pragma unroll
     for (int e = 0; e < PT; ++e) {
-      const int idx = tid + e * kThreads;
+      const int idx = tid + e * TT;
       const int R = 32 * kb + (idx >> 5), C = 32 * q + (idx & 31);
       const T g = uv[t][e];
       uv[t][e] = (R < n && C < n) ? ((R < C) ? g : ((R == C) ? T(1) / g : T(0))) : ((R == C) ? T(1) : T(0));
@@ -356,7 +362,7 @@ __global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, 
   }
 #pragma unroll
   for (int e = 0; e < PZ; ++e) {
-    const int idx = tid + e * kThreads;
+    const int idx = tid + e * TT;
     const int rr = idx / NP, kr = idx - rr * NP;
     const int kc = idx >> 5, cc = idx & 31;
     const bool ok = isR ? (rr < w && kr < n) : (cc < w && kc < n);
@@ -366,20 +372,22 @@ __global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, 
   for (int t = 0; t < NT; ++t)
 #pragma unroll
     for (int e = 0; e < PT; ++e) {
-      const int idx = tid + e * kThreads;
+      const int idx = tid + e * TT;
       Ut[t * 32 * TP + (idx >> 5) * TP + (idx & 31)] = uv[t][e];
     }
 #pragma unroll
   for (int e = 0; e < PZ; ++e) {
-    const int idx = tid + e * kThreads;
+    const int idx = tid + e * TT;
     if (isR) Xs[(idx / NP) * XP + (idx % NP)] = zv[e];
     else Xs[(idx >> 5) * TP + (idx & 31)] = zv[e];
   }
   __syncthreads();
 
-  for (int q = wave; q < NB; q += kThreads / 64) {
-    typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
-    for (int kb = 0; kb <= q; ++kb) {
+  const int q = wave & 3, half = wave >> 2;
+  const int kb0 = half ? (q + 1) / 2 : 0, kb1 = half ? q + 1 : (q + 1) / 2;
+  typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+  if (q < NB) {
+    for (int kb = kb0; kb < kb1; ++kb) {
       const T* Utile = Ut + (q * (q + 1) / 2 + kb) * 32 * TP;  // U[32kb.., 32q..]
       if (isR) {
         // out[r][c] += sum_k R[r][32kb+k] U[32kb+k][32q+c]
@@ -389,15 +397,22 @@ __global__ __launch_bounds__(kThreads) void k_trsm_blk(const T* __restrict__ L, 
         wave_gemm<T>(acc, Utile, 1, TP, Xs + 32 * kb * TP, TP, 1, 32);
       }
     }
+  }
+  // the upper half's sums go through the (now free) tiles of U
+  __syncthreads();
+  T* Red = Ut + q * 32 * TP;
+  if (q < NB && half) acc_foreach<T>(acc, [&](int r, int c, T v) { Red[r * TP + c] = v; });
+  __syncthreads();
+  if (q < NB && !half) {
     if (isR) {
       acc_foreach<T>(acc, [&](int r, int c, T v) {
         const int C = 32 * q + c;
-        if (r < w && C < n) X[(size_t)(c0 + r) * n + C] = v;
+        if (r < w && C < n) X[(size_t)(c0 + r) * n + C] = v + Red[r * TP + c];
       });
     } else {
       acc_foreach<T>(acc, [&](int r, int c, T v) {
         const int R = 32 * q + r;
-        if (c < w && R < n) X[(size_t)R * n + c0 + c] = v;
+        if (c < w && R < n) X[(size_t)R * n + c0 + c] = v + Red[r * TP + c];
       });
     }
   }
@@ -1242,8 +1257,8 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
                        n, nE, lv.s, lv.off, cpb);
   }
   else
-    hipLaunchKernelGGL((k_trsm_blk<T, NB, NF>), dim3(lv.dim, 2 * NB, B), dim3(kThreads), lds_trsm, st, L, E, mask, N,
-                       n, nE, lv.s, lv.off);
+    hipLaunchKernelGGL((k_trsm_blk<T, NB, NF>), dim3(lv.dim, 2 * NB, B), dim3(kTrsmThreads), lds_trsm, st, L, E, mask,
+                       N, n, nE, lv.s, lv.off);
   kt_end(kKcTrsm, st);
   if (after_trsm != nullptr) {
     const hipError_t e = hipEventRecord(after_trsm, st);
