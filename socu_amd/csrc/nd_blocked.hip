@@ -285,6 +285,14 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
 // longest product chain is half as long as with one wavefront per tile.
 constexpr int kTrsmThreads = 512;
 
+// (output tile q, half) of wavefront w: waves w and w + 4 share a SIMD, and
+// these pairs carry at most three of the ten tile products of NB = 4
+// (q of waves 0..7: 3 3 2 2 0 0 1 1; half: 0 1 1 0 1 0 0 1).
+__device__ __forceinline__ void trsm_wave_task(int w, int& q, int& half) {
+  q = (0x50AF >> (2 * w)) & 3;
+  half = (0x96 >> w) & 1;
+}
+
 template <typename T, int NB, int NF>
 __global__ __launch_bounds__(kTrsmThreads) void k_trsm_blk(const T* __restrict__ L, T* __restrict__ E,
                                                       const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
@@ -383,7 +391,8 @@ __global__ __launch_bounds__(kTrsmThreads) void k_trsm_blk(const T* __restrict__
   }
   __syncthreads();
 
-  const int q = wave & 3, half = wave >> 2;
+  int q, half;
+  trsm_wave_task(wave, q, half);
   const int kb0 = half ? (q + 1) / 2 : 0, kb1 = half ? q + 1 : (q + 1) / 2;
   typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
   if (q < NB) {
@@ -426,7 +435,7 @@ __global__ __launch_bounds__(kTrsmThreads) void k_trsm_blk(const T* __restrict__
 // chunks), the next chunk's loads in flight while the current one is
 // multiplied.
 template <typename T, int NB, int NF>
-__global__ __launch_bounds__(kThreads) void k_trsm_node(const T* __restrict__ L, T* __restrict__ E,
+__global__ __launch_bounds__(kTrsmThreads) void k_trsm_node(const T* __restrict__ L, T* __restrict__ E,
                                                        const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
                                                        int off, int cpb) {
   const int n = NF ? NF : n_;
@@ -434,8 +443,8 @@ __global__ __launch_bounds__(kThreads) void k_trsm_node(const T* __restrict__ L,
   constexpr int TP = 33;
   constexpr int NT = NB * (NB + 1) / 2;
   constexpr int XP = NP + 1;
-  constexpr int PT = 32 * 32 / kThreads;
-  constexpr int PZ = 32 * NP / kThreads;
+  constexpr int PT = 32 * 32 / kTrsmThreads;
+  constexpr int PZ = 32 * NP / kTrsmThreads;
   const int b = blockIdx.z;
   if (masked_out(mask, b)) return;
   const int i = s - 1 + 2 * s * blockIdx.x;
@@ -453,6 +462,7 @@ __global__ __launch_bounds__(kThreads) void k_trsm_node(const T* __restrict__ L,
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* Ut = reinterpret_cast<T*>(smem_raw);
   T* Xs = Ut + NT * 32 * TP;
+  T* Red = Xs + (32 * XP > NP * TP ? 32 * XP : NP * TP);  // 4 tiles: the upper halves' sums
   const int tid = threadIdx.x, wave = wave_id();
   // chunk q: R rows 32c.. for q < nbn (if R exists), then Lc columns 32c..
   auto chunk = [&](int q, bool& isR, int& c0, int& w, T*& X) {
@@ -465,7 +475,7 @@ __global__ __launch_bounds__(kThreads) void k_trsm_node(const T* __restrict__ L,
   auto zload = [&](T* zv, bool isR, int c0, int w, const T* X) {
 #pragma unroll
     for (int e = 0; e < PZ; ++e) {
-      const int idx = tid + e * kThreads;
+      const int idx = tid + e * kTrsmThreads;
       const int rr = idx / NP, kr = idx - rr * NP;  // R[c0+rr][kr], row reads
       const int kc = idx >> 5, cc = idx & 31;       // Lc[kc][c0+cc]
       const size_t ar = (size_t)(c0 + min(rr, w - 1)) * n + min(kr, n - 1);
@@ -476,7 +486,7 @@ __global__ __launch_bounds__(kThreads) void k_trsm_node(const T* __restrict__ L,
   auto zstore = [&](T* zv, bool isR, int w) {
 #pragma unroll
     for (int e = 0; e < PZ; ++e) {
-      const int idx = tid + e * kThreads;
+      const int idx = tid + e * kTrsmThreads;
       const int rr = idx / NP, kr = idx - rr * NP;
       const int kc = idx >> 5, cc = idx & 31;
       const bool ok = isR ? (rr < w && kr < n) : (cc < w && kc < n);
@@ -494,7 +504,7 @@ __global__ __launch_bounds__(kThreads) void k_trsm_node(const T* __restrict__ L,
     while (kb > q) { kb -= q + 1; ++q; }
 #pragma unroll
     for (int e = 0; e < PT; ++e) {
-      const int idx = tid + e * kThreads;
+      const int idx = tid + e * kTrsmThreads;
       const int R = 32 * kb + (idx >> 5), C = 32 * q + (idx & 31);
       uv[t][e] = A[(size_t)min(R, n - 1) * n + min(C, n - 1)];
     }
@@ -517,7 +527,7 @@ __global__ __launch_bounds__(kThreads) void k_trsm_node(const T* __restrict__ L,
     while (kb > q) { kb -= q + 1; ++q; }
 #pragma unroll
     for (int e = 0; e < PT; ++e) {
-      const int idx = tid + e * kThreads;
+      const int idx = tid + e * kTrsmThreads;
       const int R = 32 * kb + (idx >> 5), C = 32 * q + (idx & 31);
       const T g = uv[t][e];
       Ut[t * 32 * TP + (idx >> 5) * TP + (idx & 31)] =
@@ -536,23 +546,34 @@ __global__ __launch_bounds__(kThreads) void k_trsm_node(const T* __restrict__ L,
       chunk(q0 + q + 1, isRn, c0n, wn, Xn);
       zload(zv, isRn, c0n, wn, Xn);  // in flight during the products below
     }
-    for (int qq = wave; qq < NB; qq += kThreads / 64) {
+    {
+      // output tile qq, summed over kb <= qq by two wavefronts (see
+      // k_trsm_blk); the upper half's sums meet the lower half's in Red
+      int qq, half;
+      trsm_wave_task(wave, qq, half);
+      const int kb0 = half ? (qq + 1) / 2 : 0, kb1 = half ? qq + 1 : (qq + 1) / 2;
       typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
-      for (int kb = 0; kb <= qq; ++kb) {
-        const T* Utile = Ut + (qq * (qq + 1) / 2 + kb) * 32 * TP;  // U[32kb.., 32qq..]
-        if (isR) wave_gemm<T>(acc, Xs + 32 * kb, XP, 1, Utile, TP, 1, 32);
-        else wave_gemm<T>(acc, Utile, 1, TP, Xs + 32 * kb * TP, TP, 1, 32);
-      }
-      if (isR) {
-        acc_foreach<T>(acc, [&](int r, int c, T v) {
-          const int C = 32 * qq + c;
-          if (r < w && C < n) X[(size_t)(c0 + r) * n + C] = v;
-        });
-      } else {
-        acc_foreach<T>(acc, [&](int r, int c, T v) {
-          const int R = 32 * qq + r;
-          if (c < w && R < n) X[(size_t)R * n + c0 + c] = v;
-        });
+      if (qq < NB)
+        for (int kb = kb0; kb < kb1; ++kb) {
+          const T* Utile = Ut + (qq * (qq + 1) / 2 + kb) * 32 * TP;  // U[32kb.., 32qq..]
+          if (isR) wave_gemm<T>(acc, Xs + 32 * kb, XP, 1, Utile, TP, 1, 32);
+          else wave_gemm<T>(acc, Utile, 1, TP, Xs + 32 * kb * TP, TP, 1, 32);
+        }
+      T* Rq = Red + qq * 32 * TP;
+      if (qq < NB && half) acc_foreach<T>(acc, [&](int r, int c, T v) { Rq[r * TP + c] = v; });
+      __syncthreads();
+      if (qq < NB && !half) {
+        if (isR) {
+          acc_foreach<T>(acc, [&](int r, int c, T v) {
+            const int C = 32 * qq + c;
+            if (r < w && C < n) X[(size_t)(c0 + r) * n + C] = v + Rq[r * TP + c];
+          });
+        } else {
+          acc_foreach<T>(acc, [&](int r, int c, T v) {
+            const int R = 32 * qq + r;
+            if (c < w && R < n) X[(size_t)R * n + c0 + c] = v + Rq[r * TP + c];
+          });
+        }
       }
     }
     if (more) {
@@ -1109,6 +1130,12 @@ constexpr size_t trsm_blk_lds() {
                                                                              : (size_t)(32 * NB) * 33));
 }
 
+// k_trsm_node: k_trsm_blk's LDS and four 32 x 33 tiles for the half sums
+template <typename T, int NB>
+constexpr size_t trsm_node_lds() {
+  return trsm_blk_lds<T, NB>() + sizeof(T) * 4 * 32 * 33;
+}
+
 template <typename T, int NB>
 static hipError_t set_blk_attrs() {
   hipError_t e;
@@ -1125,10 +1152,10 @@ static hipError_t set_blk_attrs() {
                           (int)trsm_blk_lds<T, NB>());
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute((const void*)k_trsm_node<T, NB, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)trsm_blk_lds<T, NB>());
+                          (int)trsm_node_lds<T, NB>());
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void*)k_trsm_node<T, NB, 32 * NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)trsm_blk_lds<T, NB>());
+                             (int)trsm_node_lds<T, NB>());
 }
 
 // Whether a level's trsm takes k_trsm_node, and with how many chunks per
@@ -1253,8 +1280,9 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
   if (B * lv.dim >= trsm_node_min(NB)) {
     const int cpb = trsm_chunks_per_block();
     const int groups = (2 * NB + cpb - 1) / cpb;
-    hipLaunchKernelGGL((k_trsm_node<T, NB, NF>), dim3(lv.dim, groups, B), dim3(kThreads), lds_trsm, st, L, E, mask, N,
-                       n, nE, lv.s, lv.off, cpb);
+    const size_t lds_node = trsm_node_lds<T, NB>();
+    hipLaunchKernelGGL((k_trsm_node<T, NB, NF>), dim3(lv.dim, groups, B), dim3(kTrsmThreads), lds_node, st, L, E, mask,
+                       N, n, nE, lv.s, lv.off, cpb);
   }
   else
     hipLaunchKernelGGL((k_trsm_blk<T, NB, NF>), dim3(lv.dim, 2 * NB, B), dim3(kTrsmThreads), lds_trsm, st, L, E, mask,
