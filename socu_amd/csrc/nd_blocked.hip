@@ -292,6 +292,10 @@ __device__ __forceinline__ void trsm_wave_task(int w, int& q, int& half) {
   q = (0x50AF >> (2 * w)) & 3;
   half = (0x96 >> w) & 1;
 }
+// k_trsm_blk splits only NB = 4 (n > 96); smaller blocks keep one wavefront
+// per output tile (with NB <= 2 most of eight wavefronts would idle, and
+// their workgroups, many per CU in batches, would hold twice the registers).
+constexpr int trsm_blk_threads(int nb) { return nb == 4 ? kTrsmThreads : kThreads; }
 
 template <typename T, int NB, int NF>
 __global__ __launch_bounds__(kTrsmThreads) void k_trsm_blk(const T* __restrict__ L, T* __restrict__ E,
@@ -325,7 +329,8 @@ __global__ __launch_bounds__(kTrsmThreads) void k_trsm_blk(const T* __restrict__
   // (All loads are unconditional from clamped addresses and pinned only
   // after the last one is issued, so they are in flight together; the
   // masking and reciprocals follow.)
-  constexpr int TT = kTrsmThreads;
+  constexpr int TT = trsm_blk_threads(NB);
+  constexpr bool SPLIT = TT == kTrsmThreads;
   constexpr int PT = 32 * 32 / TT;
   T uv[NT][PT];
 #pragma unroll
@@ -391,9 +396,9 @@ __global__ __launch_bounds__(kTrsmThreads) void k_trsm_blk(const T* __restrict__
   }
   __syncthreads();
 
-  int q, half;
-  trsm_wave_task(wave, q, half);
-  const int kb0 = half ? (q + 1) / 2 : 0, kb1 = half ? q + 1 : (q + 1) / 2;
+  int q = wave, half = 0;
+  if constexpr (SPLIT) trsm_wave_task(wave, q, half);
+  const int kb0 = half ? (q + 1) / 2 : 0, kb1 = (half || !SPLIT) ? q + 1 : (q + 1) / 2;
   typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
   if (q < NB) {
     for (int kb = kb0; kb < kb1; ++kb) {
@@ -416,12 +421,12 @@ __global__ __launch_bounds__(kTrsmThreads) void k_trsm_blk(const T* __restrict__
     if (isR) {
       acc_foreach<T>(acc, [&](int r, int c, T v) {
         const int C = 32 * q + c;
-        if (r < w && C < n) X[(size_t)(c0 + r) * n + C] = v + Red[r * TP + c];
+        if (r < w && C < n) X[(size_t)(c0 + r) * n + C] = SPLIT ? v + Red[r * TP + c] : v;
       });
     } else {
       acc_foreach<T>(acc, [&](int r, int c, T v) {
         const int R = 32 * q + r;
-        if (c < w && R < n) X[(size_t)R * n + c0 + c] = v + Red[r * TP + c];
+        if (c < w && R < n) X[(size_t)R * n + c0 + c] = SPLIT ? v + Red[r * TP + c] : v;
       });
     }
   }
@@ -1285,8 +1290,8 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
                        N, n, nE, lv.s, lv.off, cpb);
   }
   else
-    hipLaunchKernelGGL((k_trsm_blk<T, NB, NF>), dim3(lv.dim, 2 * NB, B), dim3(kTrsmThreads), lds_trsm, st, L, E, mask,
-                       N, n, nE, lv.s, lv.off);
+    hipLaunchKernelGGL((k_trsm_blk<T, NB, NF>), dim3(lv.dim, 2 * NB, B), dim3(trsm_blk_threads(NB)), lds_trsm, st, L, E,
+                       mask, N, n, nE, lv.s, lv.off);
   kt_end(kKcTrsm, st);
   if (after_trsm != nullptr) {
     const hipError_t e = hipEventRecord(after_trsm, st);
