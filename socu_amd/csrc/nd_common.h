@@ -70,19 +70,18 @@ __host__ inline Level level_info(int N, int l) {
 // the same instruction (identical results for normal arguments); a pivot
 // below 2^-126 means a numerically singular block anyway.
 __device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
-// v_rsq_f64 (an estimate) refined by two Newton-Raphson steps,
-// y <- y + y (1 - x y^2) / 2, each of which doubles the correct bits.  The
-// pivots here are positive and normal; rsqrt(double) spends six selects per
-// call on zero, infinite and denormal arguments, on the pivot chain.  A
-// non-positive or NaN pivot still gives NaN (zero: inf * 0).
+// v_rsq_f64 (an estimate) refined by one third-order step (below), which
+// triples the correct bits.  The pivots here are positive and normal;
+// rsqrt(double) spends six selects per call on zero, infinite and denormal
+// arguments, on the pivot chain.  A non-positive or NaN pivot still gives
+// NaN (zero: inf * 0).
 __device__ __forceinline__ double rsqrt_t(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-#pragma unroll
-  for (int it = 0; it < 2; ++it) {
-    const double e = fma(-x * y, y, 1.0);
-    y = fma(0.5 * y, e, y);
-  }
-  return y;
+  // one third-order correction of the hardware estimate y: with
+  // e = 1 - x y^2, 1/sqrt(x) = y (1 - e)^(-1/2) = y (1 + e/2 + 3e^2/8 + ...),
+  // and the dropped terms are ~e^3 (e is ~2^-22 here)
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y, e * fma(e, 0.375, 0.5), y);
 }
 __device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
 __device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
