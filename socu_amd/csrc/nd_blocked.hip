@@ -58,6 +58,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
   T* S = reinterpret_cast<T*>(smem_raw);
   T* dinv = S + NP * P;  // 1/L[r][r] for every row
   T* col = dinv + NP;
+  T* ut = col + 128;  // 32 x 33: the last factored tile's inverse, transposed in full
   const int tid = threadIdx.x, wave = wave_id(), l = lane_id();
 
   const int nn = n * n;
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
     T* D = S + 32 * kb * P + 32 * kb;
     if (wave == 0) {
       if (kb > 0) trail(kb - 1, kb, kb);
-      tile_chol_inv<T>(D, P, dinv + 32 * kb, col);
+      tile_chol_inv<T>(D, P, dinv + 32 * kb, col, ut);
     } else {
       for (int j = wave - 1; j < NB - 1; j += WS) {
         if (j < kb - 1) inv_finish(kb - 1, j);
@@ -231,10 +232,10 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
       for (int k = 0; k < 32; k += 4) {
         const int kk = k + kq;
         const T a0 = X[li * P + kk], a1 = X[(li + 16) * P + kk];
-        // B(k, c) = Linv[c][k]: strict upper of D holds it for c > k
+        // B(k, c) = Linv[c][k] = ut[k][c]
         const int c0 = li, c1 = li + 16;
-        const T b0 = (c0 > kk) ? D[kk * P + c0] : ((c0 == kk) ? dinv[32 * kb + kk] : T(0));
-        const T b1 = (c1 > kk) ? D[kk * P + c1] : ((c1 == kk) ? dinv[32 * kb + kk] : T(0));
+        const T b0 = ut[kk * 33 + c0];
+        const T b1 = ut[kk * 33 + c1];
         acc[0][0] = Mfma<T>::mma(a0, b0, acc[0][0]);
         acc[0][1] = Mfma<T>::mma(a0, b1, acc[0][1]);
         acc[1][0] = Mfma<T>::mma(a1, b0, acc[1][0]);
@@ -1125,7 +1126,7 @@ __global__ __launch_bounds__(kThreads) void k_apply_bwd(const T* __restrict__ L,
 // ---------------------------------------------------------------------------
 template <typename T, int NB>
 constexpr size_t potrf_blk_lds() {
-  return sizeof(T) * ((size_t)(32 * NB) * (32 * NB + 1) + 32 * NB + 128);
+  return sizeof(T) * ((size_t)(32 * NB) * (32 * NB + 1) + 32 * NB + 128 + 32 * 33);
 }
 template <typename T, int NB>
 constexpr size_t trsm_blk_lds() {

@@ -137,6 +137,8 @@ __device__ __forceinline__ void acc16_foreach(typename Mfma<T>::acc_t& acc, F f)
 // On return the lower triangle holds L, the strict upper triangle holds the
 // transposed inverse (S[c][q] = Linv[q][c] for q > c) and dinv[r] = 1/L[r][r].
 // col: 128 entries of wave-private LDS (two buffers of two columns).
+// ut: a 32 x 33 LDS tile that receives Linv transposed in full (ut[r][c] =
+// Linv[c][r], zeros and diagonal included).
 //
 // Lanes 0-31 keep row r of the tile in u[]; lanes 32-63 keep column r of the
 // inverse (Gauss-Jordan: Linv = L_31^-1 ... L_0^-1).  The pivots are taken two
@@ -160,7 +162,7 @@ __device__ __forceinline__ void acc16_foreach(typename Mfma<T>::acc_t& acc, F f)
 // reads (not the column buffers, the pivots, nor the store), and their own
 // pivot entries are replaced explicitly.
 template <typename T>
-__device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) {
+__device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col, T* ut) {
   const int lane = lane_id();
   const int r = lane & 31;
   const bool rowlane = lane < 32;
@@ -294,6 +296,11 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col) 
   for (int c = 0; c < 32; ++c)
     if (rowlane ? (c <= r) : (c > r)) S[r * pitch + c] = u[c];
   if (rowlane) dinv[r] = rinv;
+  // ut[r][c] = Linv[c][r] in full (zeros above the diagonal, 1/L[r][r] on
+  // it): the inverse lanes' whole columns, the operand of the panel product
+  if (!rowlane)
+#pragma unroll
+    for (int c = 0; c < 32; ++c) ut[r * 33 + c] = u[c];
 }
 
 // Cholesky and inverse of a 16x16 tile by lanes 0-31 of the wavefront (row
