@@ -36,6 +36,9 @@ MEASURED_TRAFFIC = {("potrf", 128, 512, 1, "float32_f32_pivots"): 56884 * 1024,
                     ("potrf", 128, 512, 1, "float32"): 57811 * 1024}
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
+# mixed-solve factor precisions (socu_amd.mixed) and their pivot / Schur types
+PRECISIONS = {"float32": "FP32 Schur, FP64 pivots", "float32_f32_pivots": "FP32 Schur, FP32 pivots"}
 
 
 def log(msg):
@@ -53,8 +56,10 @@ def parse():
     ap.add_argument("--alpha", type=float, default=16.0)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--max-iter", type=int, default=3)
-    ap.add_argument("--precision", default="float32_f32_pivots", choices=["float32", "float32_f32_pivots"],
-                    help="FP32 factor with FP32 (default) or FP64 diagonal-block pivots; the other one is timed too")
+    ap.add_argument("--precision", default="float32", choices=list(PRECISIONS),
+                    help="factor precision of the timed mixed solve: FP32 Schur updates with FP64 diagonal-block "
+                         "pivots (float32, the default), FP32 pivots, or BF16 Schur updates; the others are timed "
+                         "too and reported beside it")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the batched and gradient workloads")
@@ -222,6 +227,95 @@ def bench_adjoint(args, dev, st):
     return res
 
 
+def path_fractions(flops, nbytes, ms):
+    """Whole-path rates of one step: algorithmic flops and bytes over the step
+    time, as fractions of MI355X's FP64, FP32 and dense BF16 MFMA peaks and of
+    HBM bandwidth."""
+    tf = flops / (ms * 1e-3) / 1e12
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"tflops": round(tf, 4), "frac_fp64_peak": round(tf / FP64_PEAK_TFLOPS, 5),
+            "frac_fp32_peak": round(tf / FP32_PEAK_TFLOPS, 5), "frac_bf16_peak": round(tf / BF16_PEAK_TFLOPS, 6),
+            "gb_per_s": round(gbs, 2), "frac_hbm": round(gbs / HBM_PEAK_GBS, 5)}
+
+
+def _stats(per):
+    return {"ms": round(float(per.mean()), 4), "p50_ms": round(float(np.median(per)), 4),
+            "p99_ms": round(float(np.percentile(per, 99)), 4)}
+
+
+def bench_fp64(L0, E0, b0, dev, st, steps, warmup, profile=True):
+    """Plain FP64 factor_and_solve of one system, graph-captured; inputs
+    restored from pristine device copies outside the timed region."""
+    import torch
+
+    from socu_amd import _lib, problems
+    from socu_amd.block_tridiag_solver import create_cholesky_factor_and_solve_launch
+
+    N, n, m = b0.shape
+    Lp, Ep, bp = (torch.tensor(a, device=dev) for a in (L0, E0, b0))
+    L, E, x = Lp.clone(), Ep.clone(), bp.clone()
+
+    def restore():
+        L.copy_(Lp)
+        E.copy_(Ep)
+        x.copy_(bp)
+
+    fs = create_cholesky_factor_and_solve_launch(L, E, x, use_cuda_graph=True)
+    per, _ = _time_steps(fs, restore, st, steps, warmup)
+    res = problems.rel_residual(L0, E0, x.cpu().numpy(), b0)
+    out = dict(_stats(per), rel_resid=res)
+    if profile:
+        lib = _lib.load()
+        ncls = len(_lib.KERNEL_CLASSES)
+        k64 = np.zeros(ncls)
+        for _ in range(3):
+            restore()
+            km, _ = _profile(lib, lambda a, b: _lib.check(lib.socu_profile_op(
+                _lib.OP_FACTOR_AND_SOLVE, L.data_ptr(), E.data_ptr(), x.data_ptr(), None, 1, N, n, m, _lib.DTYPE_F64,
+                None, ctypes.c_void_p(st.cuda_stream), a, b), "profile"), ncls)
+            k64 += km / 3
+        out["kernel_ms"] = {k: round(float(k64[i]), 4) for i, k in enumerate(_lib.KERNEL_CLASSES)}
+    return out
+
+
+def bench_mixed(L0, E0, b0, Lp, Ep, dev, st, precision, tol, max_iter, steps, warmup):
+    """Mixed-precision factor + FP64 refinement of one system (graph-captured,
+    b restored outside the timed region); returns the timing, the refinement
+    outcome and the launch (for profiling)."""
+    import torch
+
+    from socu_amd import problems
+    from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
+
+    bp = torch.tensor(b0, device=dev)
+    xm = bp.clone()
+    mx = create_cholesky_mixed_factor_and_solve_launch(Lp, Ep, xm, precision=precision, tol=tol, max_iter=max_iter,
+                                                       use_cuda_graph=True)
+    per, wall = _time_steps(mx, lambda: xm.copy_(bp), st, steps, warmup)
+    rr, its = mx.stats()
+    res = problems.rel_residual(L0, E0, xm.cpu().numpy(), b0)
+    out = dict(_stats(per), precision=precision, rel_resid=res, device_rel_resid=float(rr[0]),
+               correction_solves=int(its[0]), max_iter=max_iter,
+               wall_ms_per_step_incl_restore=round(wall * 1e3 / steps, 4))
+    return out, mx
+
+
+def bench_c2(args, dev, st):
+    """FP64 factor_and_solve at n=64, N=1024, m=16 (one system)."""
+    from socu_amd import problems
+
+    n, N, m = 64, 1024, 16
+    L0, E0, b0 = problems.readme_problem(n, N, m, alpha=10.0, seed=0)
+    r = bench_fp64(L0, E0, b0, dev, st, args.secondary_steps, 3, profile=False)
+    ff, fsv = problems.nd_flops(N, n, m)
+    res = {"workload": f"FP64 factor_and_solve, n={n} N={N} m={m} B=1, README generator alpha=10 seed 0",
+           **r, "factor_and_solve_per_s": round(1e3 / r["ms"], 2),
+           "gflops": round((ff + fsv) / (r["ms"] * 1e-3) / 1e9, 2),
+           "path": path_fractions(ff + fsv, problems.nd_bytes(N, n, m)[2], r["ms"])}
+    log(f"bench: C2 {res}")
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -232,8 +326,6 @@ def main():
     import torch
 
     from socu_amd import _lib, problems
-    from socu_amd.block_tridiag_solver import create_cholesky_factor_and_solve_launch
-    from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
 
     n, N, m = args.n, args.N, args.m
     dev = torch.device("cuda", 0)
@@ -244,116 +336,106 @@ def main():
     L0, E0, b0 = problems.readme_problem(n, N, m, alpha=args.alpha, seed=0)
     Lp = torch.tensor(L0, device=dev)
     Ep = torch.tensor(E0, device=dev)
-    bp = torch.tensor(b0, device=dev)
+    ff, fsv = problems.nd_flops(N, n, m)
+    bytes64 = problems.nd_bytes(N, n, m)[2]
 
-    # --- plain FP64 factor_and_solve (the denominator of the metric) ------
-    L, E, x = Lp.clone(), Ep.clone(), bp.clone()
+    # --- plain FP64 factor_and_solve: the denominator of the metric -------
+    r64 = bench_fp64(L0, E0, b0, dev, st, args.steps, args.warmup)
+    ms64 = r64["ms"]
+    r64["path"] = path_fractions(ff + fsv, bytes64, ms64)
+    r64["gflops"] = round((ff + fsv) / (ms64 * 1e-3) / 1e9, 2)
+    log(f"bench: fp64 factor_and_solve {r64}")
 
-    def restore64():
-        L.copy_(Lp)
-        E.copy_(Ep)
-        x.copy_(bp)
-
-    fs = create_cholesky_factor_and_solve_launch(L, E, x, use_cuda_graph=True)
-    per64, wall64 = _time_steps(fs, restore64, st, args.steps, args.warmup)
-    ms64 = float(per64.mean())
-    res64 = problems.rel_residual(L0, E0, x.cpu().numpy(), b0)
-    log(f"bench: fp64 factor_and_solve {ms64:.3f} ms (p50 {np.median(per64):.3f} p99 "
-        f"{np.percentile(per64, 99):.3f}), rel-resid {res64:.2e}")
-    k64, c64 = np.zeros(ncls), np.zeros(ncls, dtype=np.int64)
-    for _ in range(3):
-        restore64()
-        km, kc = _profile(lib, lambda a, b: _lib.check(lib.socu_profile_op(
-            _lib.OP_FACTOR_AND_SOLVE, L.data_ptr(), E.data_ptr(), x.data_ptr(), None, 1, N, n, m, _lib.DTYPE_F64,
-            ctypes.c_void_p(st.cuda_stream), a, b), "profile"), ncls)
-        k64 += km / 3
-        c64 = kc
-
-    # --- mixed FP32 factor + FP64 refinement (the metric) -----------------
-    xm = bp.clone()
-    mx = create_cholesky_mixed_factor_and_solve_launch(Lp, Ep, xm, precision=args.precision, tol=args.tol,
-                                                       max_iter=args.max_iter, use_cuda_graph=True)
-    perm, wallm = _time_steps(mx, lambda: xm.copy_(bp), st, args.steps, args.warmup)
-    msm = float(perm.mean())
-    rr, its = mx.stats()
-    resm = problems.rel_residual(L0, E0, xm.cpu().numpy(), b0)
-    log(f"bench: mixed+refine {msm:.3f} ms (p50 {np.median(perm):.3f} p99 {np.percentile(perm, 99):.3f}), "
-        f"rel-resid {resm:.2e} (device {rr[0]:.2e}) after {its[0]} correction solves")
+    # --- mixed factor + FP64 refinement: the metric -----------------------
+    rm, mx = bench_mixed(L0, E0, b0, Lp, Ep, dev, st, args.precision, args.tol, args.max_iter, args.steps,
+                         args.warmup)
+    msm = rm["ms"]
+    log(f"bench: mixed+refine ({args.precision}) {rm}")
     # the same graph captured with only the iterations this problem needs:
     # the difference is the cost of the converged (no-op) iterations
-    xm2 = bp.clone()
-    need = max(int(its[0]), 1)
-    mx2 = create_cholesky_mixed_factor_and_solve_launch(Lp, Ep, xm2, precision=args.precision, tol=args.tol,
-                                                        max_iter=need, use_cuda_graph=True)
-    perm2, _ = _time_steps(mx2, lambda: xm2.copy_(bp), st, args.steps, args.warmup)
-    ms_need = float(perm2.mean())
-    log(f"bench: mixed+refine with max_iter={need}: {ms_need:.3f} ms")
-    # the other pivot precision, same protocol (reported, not the value)
-    other = "float32" if args.precision == "float32_f32_pivots" else "float32_f32_pivots"
-    xo = bp.clone()
-    mo = create_cholesky_mixed_factor_and_solve_launch(Lp, Ep, xo, precision=other, tol=args.tol,
-                                                       max_iter=args.max_iter, use_cuda_graph=True)
-    pero, _ = _time_steps(mo, lambda: xo.copy_(bp), st, args.steps, args.warmup)
-    rro, itso = mo.stats()
-    other_rec = {"precision": other, "ms": round(float(pero.mean()), 4), "correction_solves": int(itso[0]),
-                 "rel_resid": problems.rel_residual(L0, E0, xo.cpu().numpy(), b0)}
-    log(f"bench: mixed+refine with {other}: {other_rec}")
+    need = max(rm["correction_solves"], 1)
+    rneed, _ = bench_mixed(L0, E0, b0, Lp, Ep, dev, st, args.precision, args.tol, need, args.steps, args.warmup)
+    log(f"bench: mixed+refine with max_iter={need}: {rneed['ms']:.4f} ms")
     km_mixed, kc_mixed = _profile(lib, lambda a, b: _lib.check(lib.socu_mixed_profile(
         mx._h, ctypes.c_void_p(st.cuda_stream), a, b), "mixed profile"), ncls)
-    kernel_ms = {k: float(km_mixed[i]) for i, k in enumerate(_lib.KERNEL_CLASSES)}
+    kernel_ms = {k: round(float(km_mixed[i]), 4) for i, k in enumerate(_lib.KERNEL_CLASSES)}
     kernel_launch = {k: int(kc_mixed[i]) for i, k in enumerate(_lib.KERNEL_CLASSES)}
     log(f"bench: mixed per-class kernel ms {kernel_ms} launches {kernel_launch}")
-    log(f"bench: fp64  per-class kernel ms " + str({k: round(float(k64[i]), 4) for i, k in
-                                                     enumerate(_lib.KERNEL_CLASSES)}))
+    # whole-path rates: the FP64 path's algorithmic flops and bytes plus, per
+    # correction solve, an FP32 solve and an FP64 residual
+    solves = max(rm["correction_solves"], 1)
+    cb = problems.class_bytes(N, n, m, w=4)
+    mixed_bytes = problems.nd_bytes(N, n, m, w=4)[0] + solves * (cb["forward"] + cb["backward"] + cb["refine"])
+    mixed_flops = ff + solves * (fsv + problems.block_matvec_flops(N, n, m))
+    rm["path"] = path_fractions(mixed_flops, mixed_bytes, msm)
 
-    # --- the literal alpha=10 configuration (numerically singular at n=128) --
-    literal = None
-    if args.alpha != 10.0 and n == 128:
-        L1, E1, b1 = problems.readme_problem(n, N, m, alpha=10.0, seed=0)
-        L1t, E1t, x1 = torch.tensor(L1, device=dev), torch.tensor(E1, device=dev), torch.tensor(b1, device=dev)
-        m1 = create_cholesky_mixed_factor_and_solve_launch(L1t, E1t, x1, tol=args.tol, max_iter=args.max_iter)
-        m1()
-        rr1, it1 = m1.stats()
-        xf = torch.tensor(b1, device=dev)
-        Lf, Ef = L1t.clone(), E1t.clone()
-        create_cholesky_factor_and_solve_launch(Lf, Ef, xf)()
-        literal = {"alpha": 10.0, "mixed_rel_resid": float(rr1[0]), "mixed_solves": int(it1[0]),
-                   "fp64_rel_resid": problems.rel_residual(L1, E1, xf.cpu().numpy(), b1)}
-        log(f"bench: literal alpha=10: {literal}")
+    # --- the other factor precisions, same protocol (reported beside) -----
+    variants = {}
+    for prec in PRECISIONS:
+        if prec == args.precision:
+            continue
+        rv, _ = bench_mixed(L0, E0, b0, Lp, Ep, dev, st, prec, args.tol, args.max_iter, args.steps, args.warmup)
+        rv["speedup_vs_fp64"] = round(ms64 / rv["ms"], 4)
+        variants[prec] = rv
+        log(f"bench: mixed+refine ({prec}) {rv}")
 
-    ff, fsv = problems.nd_flops(N, n, m)
-    # roofline of the largest single kernel of the measured (mixed) path: the
-    # diagonal-block factor (the forward class bundles several kernels: level
-    # updates and applies and the persistent upper-level sweep)
+    # --- C3 with eight right-hand sides ------------------------------------
+    c3_m8 = None
+    if not args.no_secondary and m == 1:
+        L8, E8, b8 = problems.readme_problem(n, N, 8, alpha=args.alpha, seed=0)
+        r8 = bench_fp64(L8, E8, b8, dev, st, args.secondary_steps, 3, profile=False)
+        c3_m8 = {"workload": f"n={n} N={N} m=8 B=1, alpha={args.alpha}", "fp64": r8}
+        for prec in PRECISIONS:
+            rv, _ = bench_mixed(L8, E8, b8, Lp, Ep, dev, st, prec, args.tol, args.max_iter, args.secondary_steps, 3)
+            rv["speedup_vs_fp64"] = round(r8["ms"] / rv["ms"], 4)
+            c3_m8[prec] = rv
+        log(f"bench: C3 m=8 {c3_m8}")
+
+    # --- the literal alphas of the configuration (numerically singular at n=128)
+    literal = {}
+    if n == 128:
+        from socu_amd.block_tridiag_solver import create_cholesky_factor_and_solve_launch
+        from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
+
+        for alpha in (10.0, 2.0):
+            L1, E1, b1 = problems.readme_problem(n, N, m, alpha=alpha, seed=0)
+            L1t, E1t, x1 = torch.tensor(L1, device=dev), torch.tensor(E1, device=dev), torch.tensor(b1, device=dev)
+            m1 = create_cholesky_mixed_factor_and_solve_launch(L1t, E1t, x1, tol=args.tol, max_iter=args.max_iter)
+            m1()
+            rr1, it1 = m1.stats()
+            xf = torch.tensor(b1, device=dev)
+            Lf, Ef = L1t.clone(), E1t.clone()
+            create_cholesky_factor_and_solve_launch(Lf, Ef, xf)()
+            diag = np.diagonal(Lf.cpu().numpy(), axis1=1, axis2=2)
+            bad = np.where(~np.isfinite(diag).all(1))[0]
+            literal[str(alpha)] = {"mixed_rel_resid": float(rr1[0]), "mixed_solves": int(it1[0]),
+                                   "fp64_rel_resid": problems.rel_residual(L1, E1, xf.cpu().numpy(), b1),
+                                   "fp64_breakdown_nodes": [int(v) for v in bad[:16]]}
+        log(f"bench: literal alphas: {literal}")
+
+    # --- roofline of the dominant kernel of the timed path -----------------
+    # k_potrf_blk (the diagonal-block factor; one launch per ND level) is the
+    # largest single kernel: algorithmic flops n^3/3 per node (the explicit
+    # inverse it also forms is not counted), over its summed launch time
+    # (HIP events around each launch in an eager run of the same solve).  With
+    # FP64 pivots it computes in FP64.
     dom = "potrf"
-    solves = max(int(its[0]), 1)
-    if dom in ("forward", "backward", "refine"):
-        # sweeps and refinement stream the factor / matrix: HBM-bound
-        per = problems.class_bytes(N, n, m, w=4)[dom] * solves
-        achieved = per / (kernel_ms[dom] * 1e-3) / 1e9 if kernel_ms[dom] > 0 else 0.0
-        bound, peak, unit = "hbm", HBM_PEAK_GBS, "GB/s"
-    else:
-        cls_flops = problems.class_flops(N, n, m)
-        per = cls_flops.get(dom, 0.0)
-        if dom == "potrf":
-            per *= 2  # factor and explicit inverse of each diagonal block
-        achieved = per / (kernel_ms[dom] * 1e-3) / 1e12 if kernel_ms[dom] > 0 else 0.0
-        # the mixed potrf computes in FP64 with FP64 pivots, else in FP32;
-        # trsm and Schur in FP32
-        bound, unit = "mfma", "TFLOP/s"
-        peak = FP64_PEAK_TFLOPS if (dom == "potrf" and args.precision == "float32") else FP32_PEAK_TFLOPS
+    per_flops = problems.class_flops(N, n, m)[dom]
+    achieved = per_flops / (kernel_ms[dom] * 1e-3) / 1e12 if kernel_ms[dom] > 0 else 0.0
+    peak = FP32_PEAK_TFLOPS if args.precision == "float32_f32_pivots" else FP64_PEAK_TFLOPS
     roofline = {
-        "kernel": dom,
-        "bound": bound,
+        "kernel": "k_potrf_blk",
+        "bound": "mfma",
         "achieved": round(achieved, 4),
         "peak": peak,
-        "unit": unit,
+        "unit": "TFLOP/s",
         "frac": round(achieved / peak, 5),
         "traffic": MEASURED_TRAFFIC.get((dom, n, N, m, args.precision)),
-        "traffic_algorithmic_bytes": N * 2 * n * n * 4 if dom == "potrf" else None,
-        "algorithmic_per_step": per,
+        "traffic_algorithmic_bytes": N * n * (n + 1) // 2 * 4 + N * n * n * 4,
+        "algorithmic_per_step": per_flops,
         "launches_per_step": kernel_launch[dom],
-        "ms_per_step": round(kernel_ms[dom], 4),
+        "avg_launch_us": round(kernel_ms[dom] * 1e3 / max(kernel_launch[dom], 1), 2),
+        "ms_per_step": kernel_ms[dom],
     }
 
     cpu = None
@@ -366,9 +448,10 @@ def main():
 
     secondary = None
     if not args.no_secondary:
-        secondary = {"mpc_batched": bench_mpc_batch(args, dev, st), "adjoint": bench_adjoint(args, dev, st)}
+        secondary = {"c2_fp64": bench_c2(args, dev, st), "c3_m8": c3_m8,
+                     "mpc_batched": bench_mpc_batch(args, dev, st), "adjoint": bench_adjoint(args, dev, st)}
 
-    mixed_ok = resm <= args.tol
+    mixed_ok = rm["rel_resid"] <= args.tol
     value = 1e3 / msm if mixed_ok else 0.0
     out = {
         "metric": "factor+solve/s, mixed+refine vs FP64, n=128 N=512 @ rel-resid 1e-10",
@@ -381,22 +464,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 factor" + (" (f32 pivots)" if args.precision == "float32_f32_pivots" else " (f64 pivots)")
-                 + " + f64 refinement",
+        "dtype": {"float32": "f32 Schur + f64 pivots, f64 refinement",
+                  "float32_f32_pivots": "f32 factor (f32 pivots), f64 refinement"}[args.precision],
         "data": "synthetic (README generator, seed 0)",
-        "config": {"workload": f"ND block-tridiagonal Cholesky, mixed FP32 factor + FP64 refinement to rel-resid "
-                               f"{args.tol:g}, n={n} N={N} m={m} B=1, alpha={args.alpha}",
-                   "n": n, "N": N, "m": m, "alpha": args.alpha,
-                   "mixed_rel_resid": resm, "mixed_correction_solves": int(its[0]),
-                   "fp64_factor_and_solve_per_s": round(1e3 / ms64, 3), "fp64_ms": round(ms64, 4),
-                   "fp64_rel_resid": res64, "speedup_vs_fp64": round(ms64 / msm, 4),
-                   "mixed_ms_with_max_iter_needed": round(ms_need, 4), "max_iter": args.max_iter,
-                   "precision": args.precision, "other_precision": other_rec,
-                   "fp64_gflops": round((ff + fsv) / (ms64 * 1e-3) / 1e9, 2),
-                   "kernel_ms": kernel_ms, "fp64_kernel_ms": {k: round(float(k64[i]), 4)
-                                                             for i, k in enumerate(_lib.KERNEL_CLASSES)},
-                   "literal_alpha10": literal,
-                   "wall_ms_per_step_incl_restore": round(wallm * 1e3 / args.steps, 4)},
+        "config": {"workload": f"ND block-tridiagonal Cholesky, mixed factor ({PRECISIONS[args.precision]}) + FP64 "
+                               f"refinement to rel-resid {args.tol:g}, n={n} N={N} m={m} B=1, alpha={args.alpha} "
+                               "(the literal alpha=10 and alpha=2 are numerically singular at n=128: "
+                               "literal_alpha)",
+                   "n": n, "N": N, "m": m, "alpha": args.alpha, "precision": args.precision,
+                   "mixed": rm, "mixed_ms_with_max_iter_needed": rneed["ms"],
+                   "speedup_vs_fp64": round(ms64 / msm, 4),
+                   "fp64_factor_and_solve_per_s": round(1e3 / ms64, 3), "fp64": r64,
+                   "variants": variants, "kernel_ms": kernel_ms, "kernel_launches": kernel_launch,
+                   "literal_alpha": literal},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "secondary": secondary,

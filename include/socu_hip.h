@@ -26,6 +26,8 @@
 #ifndef SOCU_HIP_H
 #define SOCU_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -37,6 +39,7 @@ extern "C" {
 #define SOCU_EINVAL -1      /* bad argument (shape, dtype, pointer) */
 #define SOCU_EUNSUPPORTED -2 /* valid but not supported (e.g. n > max block size) */
 #define SOCU_EHIP -3        /* a HIP runtime call failed */
+#define SOCU_EEXPIRED -4    /* a bounded wait expired in an earlier execution of this plan / handle */
 
 /* Operations a plan can hold (the five launch factories). */
 #define SOCU_OP_FACTOR 0           /* create_cholesky_factor_launch, block_tridiag_solver.py:739-923 */
@@ -60,19 +63,30 @@ int socu_max_block_size(void);
  * (block_tridiag_solver.py:864-918).  E[b, N-1:] must be zero on entry. */
 int socu_cholesky_factor(void* L, void* E, const unsigned char* mask, int B, int N, int n, int dtype, void* stream);
 
+/* Substitutions read only tril(L) and E (as the reference's sweeps do,
+ * socu/blas/potrf.py:101-103), so the factor may come from anywhere.  They
+ * first form the inverses of the diagonal factor blocks from tril(L) into
+ * `work`, a device buffer of socu_solve_workspace_size(B, N, n, dtype) bytes
+ * (one n x n block per diagonal block), which they own for the duration of
+ * the call; L and E are not modified.  work must not be NULL when B > 0. */
+size_t socu_solve_workspace_size(int B, int N, int n, int dtype);
+
 /* x <- Lp^{-1} P x (forward sweep), block_tridiag_solver.py:1339-1369. */
 int socu_cholesky_forward_substitution(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N,
-                                       int n, int m, int dtype, void* stream);
+                                       int n, int m, int dtype, void* work, void* stream);
 
 /* x <- P^T Lp^{-T} x (backward sweep), block_tridiag_solver.py:1480-1512. */
 int socu_cholesky_backward_substitution(const void* L, const void* E, void* x, const unsigned char* mask, int B,
-                                        int N, int n, int m, int dtype, void* stream);
+                                        int N, int n, int m, int dtype, void* work, void* stream);
 
 /* forward then backward sweep, block_tridiag_solver.py:1666-1722. */
 int socu_cholesky_solve(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
-                        int dtype, void* stream);
+                        int dtype, void* work, void* stream);
 
-/* factor, forward, backward, block_tridiag_solver.py:1124-1228. */
+/* factor, forward, backward, block_tridiag_solver.py:1124-1228.  The
+ * factor leaves the transposed inverse of each diagonal factor block in the
+ * strict upper triangle of L[b, i] (the reference leaves that triangle
+ * implementation-defined), and the sweeps of the same call read it there. */
 int socu_cholesky_factor_and_solve(void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
                                    int dtype, void* stream);
 
@@ -80,7 +94,8 @@ int socu_cholesky_factor_and_solve(void* L, void* E, void* x, const unsigned cha
  * callables (arrays bound at creation).  With use_graph != 0 the work is
  * captured once into a hipGraph on a private capture stream and replayed by
  * socu_plan_execute (replaces utils.py:40-66, create_cuda_graph_callback).
- * x may be NULL for SOCU_OP_FACTOR. */
+ * x may be NULL for SOCU_OP_FACTOR.  Solve, forward and backward plans own
+ * the diagonal-block inverse workspace of the direct entry points. */
 typedef struct socu_plan socu_plan;
 int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N,
                      int n, int m, int dtype, int use_graph);
@@ -88,20 +103,30 @@ int socu_plan_execute(socu_plan* plan, void* stream);
 int socu_plan_destroy(socu_plan* plan);
 
 /* Substitution sweeps with one right-hand side (m == 1): a plan (and a mixed
- * handle) may own a small workspace and run the upper ND levels of forward
- * and backward substitution as one persistent launch of node tasks that wait
- * on per-node counters; the wide lower levels keep the level kernels (same
+ * handle) owns a small workspace and runs the upper ND levels of forward and
+ * backward substitution as one persistent launch of node tasks that wait on
+ * per-node counters; the wide lower levels keep the level kernels (same
  * arithmetic per block as the reference's sweeps,
- * block_tridiag_solver.py:425-485 and :581-643).  Chosen where it measured
- * faster (FP32 blocks with n > 96); the environment variable SOCU_SWEEP read
- * at creation overrides: "level" (never) or "persistent" (always).  Each wait
- * is bounded; if one expires the launch drains with undefined results and
- * records it.  socu_plan_sweep_status synchronises `stream` and reports
+ * block_tridiag_solver.py:425-485 and :581-643).  Used by every plan with one
+ * right-hand side, in both precisions (it measured faster than the level
+ * kernels for every block size); the environment variable SOCU_SWEEP read at
+ * creation overrides: "level" (never) or "persistent" (always).  Each wait
+ * is bounded; if one expires the launch drains, the tasks that depended on
+ * the wait write NaN into their part of x (so the failure shows in the
+ * solution), and a host-visible word is raised: the plan's next
+ * socu_plan_execute then returns SOCU_EEXPIRED without running (and clears
+ * the word).  socu_plan_sweep_status synchronises `stream` and reports
  * whether the plan uses the persistent sweeps (*persistent) and whether a
- * wait expired in its last execution (*expired).  Executions of one plan must
- * not overlap (they share the workspace).  (No counterpart in the
- * reference.) */
+ * wait expired in its last execution (*expired; this also clears the word).
+ * Executions of one plan must not overlap (they share the workspace).  (No
+ * counterpart in the reference.) */
 int socu_plan_sweep_status(socu_plan* plan, int* persistent, int* expired, void* stream);
+/* Polls per dependency wait of the persistent sweeps enqueued from now on
+ * (process-wide; a captured graph keeps the value it was captured with).
+ * 0 restores the default (2^21 polls); a negative value makes every wait that
+ * is not already satisfied expire at once.  For tests of the failure path:
+ * every kernel still exits normally.  (No counterpart in the reference.) */
+int socu_set_sweep_spin_limit(int polls);
 
 /* Mixed-precision factor_and_solve with FP64 iterative refinement (no
  * counterpart in the reference; an extension with the same data layout).
@@ -124,7 +149,9 @@ int socu_mixed_create(socu_mixed** out, const void* L, const void* E, void* x, c
 int socu_mixed_execute(socu_mixed* handle, void* stream);
 int socu_mixed_stats(socu_mixed* handle, double* rel_resid, int* solves, void* stream);
 int socu_mixed_destroy(socu_mixed* handle);
-/* As socu_plan_sweep_status, for the FP32 correction solves of a handle. */
+/* As socu_plan_sweep_status, for the FP32 correction solves of a handle
+ * (socu_mixed_execute likewise returns SOCU_EEXPIRED after an execution whose
+ * wait expired). */
 int socu_mixed_sweep_status(socu_mixed* handle, int* persistent, int* expired, void* stream);
 /* Runs the mixed solve once without the graph, with the per-kernel-class
  * event timer of socu_profile_op (classes 0-4 as there, 5 = FP64 residual and
@@ -140,7 +167,7 @@ int socu_mixed_profile(socu_mixed* handle, void* stream, double* kernel_ms, int*
  * for measurement, not for hot loops. */
 #define SOCU_KERNEL_CLASSES 6
 int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
-                    int dtype, void* stream, double* kernel_ms, int* kernel_launches);
+                    int dtype, void* work, void* stream, double* kernel_ms, int* kernel_launches);
 
 /* Batched uneven systems (no counterpart in the reference, whose launches
  * batch only systems sharing N and n: block_tridiag_solver.py:749-768).

@@ -38,6 +38,7 @@ __all__ = [
     "level_schedule",
     "e_index",
     "nd_order",
+    "readme_batch",
     "readme_problem",
     "reference_test_problem",
     "dense_from_blocks",
@@ -143,6 +144,38 @@ def readme_problem(n: int, N: int, m: int = 1, alpha: float = 10.0, seed: int = 
     if N > 1:
         E[: N - 1] = np.einsum("nij,nkj->nik", Ec, D[: N - 1])
     return L, E, b, D, Ec
+
+
+def readme_batch(count: int, n: int, N: int, m: int = 1, alpha: float = 10.0, seed: int = 0,
+                 with_weights: bool = False):
+    """``count`` systems of the README generator drawn from one stream
+    RandomState(seed), each system's draws in turn (the generator's own order:
+    D_i, Ec_i interleaved, then b), then, if ``with_weights``, a loss weight w
+    of b's shape per system.  Returns stacked (L, E, b[, w]) with a leading
+    batch axis."""
+    rs = np.random.RandomState(seed)
+    Ls, Es, bs = [], [], []
+    for _ in range(count):
+        D = np.zeros((N, n, n))
+        Ec = np.zeros((max(N - 1, 0), n, n))
+        for i in range(N):
+            D[i] = np.tril(rs.randn(n, n)) + alpha * np.eye(n)
+            if i < N - 1:
+                Ec[i] = rs.randn(n, n)
+        b = rs.randn(N, n, m)
+        L = np.einsum("nij,nkj->nik", D, D)
+        if N > 1:
+            L[1:] += np.einsum("nij,nkj->nik", Ec, Ec)
+        E = np.zeros((calculate_off_diag_storage_len(N), n, n))
+        if N > 1:
+            E[: N - 1] = np.einsum("nij,nkj->nik", Ec, D[: N - 1])
+        Ls.append(L)
+        Es.append(E)
+        bs.append(b)
+    out = [np.stack(Ls), np.stack(Es), np.stack(bs)]
+    if with_weights:
+        out.append(rs.randn(count, N, n, m))
+    return tuple(out)
 
 
 def reference_test_problem(n: int, N: int, seed: int = 42):

@@ -22,6 +22,7 @@ OK = 0
 EINVAL = -1
 EUNSUPPORTED = -2
 EHIP = -3
+EEXPIRED = -4
 
 OP_FACTOR = 0
 OP_SOLVE = 1
@@ -49,16 +50,18 @@ SIGNATURES = {
     "socu_calculate_recursive_iterations": (_i, [_i]),
     "socu_calculate_off_diag_storage_len": (ctypes.c_longlong, [_i]),
     "socu_max_block_size": (_i, []),
+    "socu_set_sweep_spin_limit": (_i, [_i]),
+    "socu_solve_workspace_size": (ctypes.c_size_t, [_i, _i, _i, _i]),
     "socu_cholesky_factor": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _vp]),
-    "socu_cholesky_forward_substitution": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
-    "socu_cholesky_backward_substitution": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
-    "socu_cholesky_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "socu_cholesky_forward_substitution": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp]),
+    "socu_cholesky_backward_substitution": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp]),
+    "socu_cholesky_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp]),
     "socu_cholesky_factor_and_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "socu_plan_create": (_i, [ctypes.POINTER(_vp), _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i]),
     "socu_plan_execute": (_i, [_vp, _vp]),
     "socu_plan_destroy": (_i, [_vp]),
     "socu_plan_sweep_status": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _vp]),
-    "socu_profile_op": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp,
+    "socu_profile_op": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp,
                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
     "socu_mixed_create": (_i, [ctypes.POINTER(_vp), _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, ctypes.c_double, _i, _i]),
     "socu_mixed_execute": (_i, [_vp, _vp]),
@@ -80,6 +83,11 @@ _lib = None
 
 class SocuError(RuntimeError):
     """A call into libsocu_hip.so returned an error code."""
+
+
+class SweepExpiredError(SocuError):
+    """A bounded dependency wait of a persistent substitution sweep expired
+    in an earlier call of the launch; that call's solution holds NaN."""
 
 
 def load():
@@ -113,4 +121,6 @@ def check(rc: int, what: str):
             raise ValueError(f"{what}: {msg}")
         if rc == EUNSUPPORTED:
             raise NotImplementedError(f"{what}: {msg}")
+        if rc == EEXPIRED:
+            raise SweepExpiredError(f"{what}: {msg}")
         raise SocuError(f"{what}: {msg} (code {rc})")

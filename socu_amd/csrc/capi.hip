@@ -93,13 +93,16 @@ int run_backward(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N
 struct SweepWs {
   int* ws = nullptr;
   void* slots = nullptr;
+  int* hstatus = nullptr;      // pinned host word an expired wait raises
+  int* hstatus_dev = nullptr;  // its device address
 };
 
 template <typename T>
 int run_sweeps(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m, bool fwd, bool bwd,
                const SweepWs& sw, hipStream_t st) {
   if (sw.ws != nullptr && m == 1) {
-    hipError_t e = socu::sweep_persistent<T>(L, E, x, mask, B, N, n, fwd, bwd, sw.ws, static_cast<T*>(sw.slots), st);
+    hipError_t e = socu::sweep_persistent<T>(L, E, x, mask, B, N, n, fwd, bwd, sw.ws, static_cast<T*>(sw.slots),
+                                             sw.hstatus_dev, st);
     if (e != hipSuccess) return hip_fail(e, "persistent sweep launch");
     return SOCU_OK;
   }
@@ -111,11 +114,21 @@ int run_sweeps(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, 
 
 template <typename T>
 int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N, int n, int m, const SweepWs& sw,
-           const socu::SweepOverlap* ov, hipStream_t st) {
+           const socu::SweepOverlap* ov, void* inv, hipStream_t st) {
   T* Lt = static_cast<T*>(L);
   T* Et = static_cast<T*>(E);
   T* xt = static_cast<T*>(x);
   int rc = SOCU_OK;
+  if (op == SOCU_OP_FORWARD || op == SOCU_OP_SOLVE || op == SOCU_OP_BACKWARD) {
+    // The sweeps multiply by the inverses of the diagonal factor blocks;
+    // they are formed here from tril(L), so the factor may come from
+    // anywhere (only tril(L) is read, as in the reference), and the solve
+    // reads them from the workspace in place of L.
+    T* W = static_cast<T*>(inv);
+    const hipError_t e = socu::trtri_blocked<T>(Lt, W, mask, B, N, n, st);
+    if (e != hipSuccess) return hip_fail(e, "diagonal-block inverse launch");
+    Lt = W;
+  }
   switch (op) {
     case SOCU_OP_FACTOR:
       return run_factor<T>(Lt, Et, mask, B, N, n, st);
@@ -152,17 +165,28 @@ int ensure_init() {
   return SOCU_OK;
 }
 
+// Bytes of the diagonal-block inverse workspace of a solve, forward or
+// backward substitution: one n x n block per diagonal block.
+size_t inverse_ws_bytes(int B, int N, int n, int dtype) {
+  const size_t es = dtype == SOCU_DTYPE_F64 ? sizeof(double) : sizeof(float);
+  return (size_t)B * N * n * n * es;
+}
+
+bool op_needs_inverse(int op) { return op == SOCU_OP_FORWARD || op == SOCU_OP_SOLVE || op == SOCU_OP_BACKWARD; }
+
 int dispatch(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m, int dtype,
-             void* stream, const SweepWs& sw = SweepWs(), const socu::SweepOverlap* ov = nullptr) {
+             void* stream, void* inv, const SweepWs& sw = SweepWs(), const socu::SweepOverlap* ov = nullptr) {
   const bool need_x = op != SOCU_OP_FACTOR;
   int rc = check_dims(L, E, x, need_x, B, N, n, m, dtype);
   if (rc) return rc;
   if (B == 0) return SOCU_OK;
+  if (op_needs_inverse(op) && inv == nullptr)
+    return fail(SOCU_EINVAL, "work is NULL: substitutions need socu_solve_workspace_size() bytes of device workspace");
   rc = ensure_init();
   if (rc) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == SOCU_DTYPE_F64) return run_op<double>(op, L, E, x, mask, B, N, n, m, sw, ov, st);
-  return run_op<float>(op, L, E, x, mask, B, N, n, m, sw, ov, st);
+  if (dtype == SOCU_DTYPE_F64) return run_op<double>(op, L, E, x, mask, B, N, n, m, sw, ov, inv, st);
+  return run_op<float>(op, L, E, x, mask, B, N, n, m, sw, ov, inv, st);
 }
 
 // Per-thread launch timer used by socu_profile_op.
@@ -208,6 +232,7 @@ struct socu_plan {
   SweepWs sweep;  // persistent-sweep workspace (m == 1 solves), else NULL
   socu::SweepOverlap overlap;  // factor_and_solve: forward sweep alongside the factor
   bool has_overlap;
+  void* inv;  // solve / forward / backward: diagonal-block inverses formed from tril(L)
 };
 
 namespace {
@@ -215,6 +240,7 @@ namespace {
 void free_sweep_ws(SweepWs& sw) {
   if (sw.ws) (void)hipFree(sw.ws);
   if (sw.slots) (void)hipFree(sw.slots);
+  if (sw.hstatus) (void)hipHostFree(sw.hstatus);
   sw = SweepWs();
 }
 
@@ -226,6 +252,11 @@ int alloc_sweep_ws(SweepWs& sw, int op, int B, int N, int n, int m, int dtype) {
   hipError_t e = hipMalloc(&sw.ws, sizeof(int) * socu::sweep_ws_ints(B, N));
   if (e == hipSuccess) e = hipMemset(sw.ws, 0, sizeof(int) * socu::sweep_ws_ints(B, N));  // status reads 0 before a run
   if (e == hipSuccess) e = hipMalloc(&sw.slots, esz * socu::sweep_slot_elems(B, N, n));
+  if (e == hipSuccess) e = hipHostMalloc(&sw.hstatus, sizeof(int), hipHostMallocMapped);
+  if (e == hipSuccess) {
+    *sw.hstatus = 0;
+    e = hipHostGetDevicePointer((void**)&sw.hstatus_dev, sw.hstatus, 0);
+  }
   if (e != hipSuccess) {
     free_sweep_ws(sw);
     return hip_fail(e, "hipMalloc (sweep workspace)");
@@ -233,7 +264,7 @@ int alloc_sweep_ws(SweepWs& sw, int op, int B, int N, int n, int m, int dtype) {
   return SOCU_OK;
 }
 
-int sweep_status(int* ws, int* persistent, int* expired, void* stream) {
+int sweep_status(int* ws, int* hstatus, int* persistent, int* expired, void* stream) {
   if (persistent == nullptr || expired == nullptr) return fail(SOCU_EINVAL, "output pointers are NULL");
   *persistent = ws != nullptr ? 1 : 0;
   *expired = 0;
@@ -243,7 +274,19 @@ int sweep_status(int* ws, int* persistent, int* expired, void* stream) {
   if (e == hipSuccess) e = hipMemcpy(&st, ws + 1, sizeof(int), hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_fail(e, "reading the sweep status");
   *expired = st != 0 ? 1 : 0;
+  if (hstatus != nullptr) __atomic_store_n(hstatus, 0, __ATOMIC_SEQ_CST);  // reported here
   return SOCU_OK;
+}
+
+// An execution whose persistent sweep had a wait expire left NaN in its
+// solution and raised the host-visible word; the next execution reports it
+// (and clears the word) instead of running.
+int check_expired(int* hstatus) {
+  if (hstatus == nullptr || __atomic_load_n(hstatus, __ATOMIC_SEQ_CST) == 0) return SOCU_OK;
+  __atomic_store_n(hstatus, 0, __ATOMIC_SEQ_CST);
+  return fail(SOCU_EEXPIRED,
+              "a dependency wait of the persistent substitution sweep expired in an earlier call of this launch; "
+              "that call's solution holds NaN");
 }
 
 }  // namespace
@@ -262,30 +305,43 @@ long long socu_calculate_off_diag_storage_len(int N) {
 
 int socu_max_block_size(void) { return socu::kMaxN; }
 
+int socu_set_sweep_spin_limit(int polls) {
+  socu::set_sweep_spin_limit(polls);
+  return SOCU_OK;
+}
+
 const char* socu_last_error(void) { return g_last_error.c_str(); }
 
+size_t socu_solve_workspace_size(int B, int N, int n, int dtype) {
+  if (B < 0 || N < 1 || n < 1 || (dtype != SOCU_DTYPE_F32 && dtype != SOCU_DTYPE_F64)) return 0;
+  return inverse_ws_bytes(B, N, n, dtype);
+}
+
 int socu_cholesky_factor(void* L, void* E, const unsigned char* mask, int B, int N, int n, int dtype, void* stream) {
-  return dispatch(SOCU_OP_FACTOR, L, E, nullptr, mask, B, N, n, 1, dtype, stream);
+  return dispatch(SOCU_OP_FACTOR, L, E, nullptr, mask, B, N, n, 1, dtype, stream, nullptr);
 }
 
 int socu_cholesky_forward_substitution(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N,
-                                       int n, int m, int dtype, void* stream) {
-  return dispatch(SOCU_OP_FORWARD, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream);
+                                       int n, int m, int dtype, void* work, void* stream) {
+  return dispatch(SOCU_OP_FORWARD, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream,
+                  work);
 }
 
 int socu_cholesky_backward_substitution(const void* L, const void* E, void* x, const unsigned char* mask, int B,
-                                        int N, int n, int m, int dtype, void* stream) {
-  return dispatch(SOCU_OP_BACKWARD, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream);
+                                        int N, int n, int m, int dtype, void* work, void* stream) {
+  return dispatch(SOCU_OP_BACKWARD, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream,
+                  work);
 }
 
 int socu_cholesky_solve(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
-                        int dtype, void* stream) {
-  return dispatch(SOCU_OP_SOLVE, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream);
+                        int dtype, void* work, void* stream) {
+  return dispatch(SOCU_OP_SOLVE, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream,
+                  work);
 }
 
 int socu_cholesky_factor_and_solve(void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
                                    int dtype, void* stream) {
-  return dispatch(SOCU_OP_FACTOR_AND_SOLVE, L, E, x, mask, B, N, n, m, dtype, stream);
+  return dispatch(SOCU_OP_FACTOR_AND_SOLVE, L, E, x, mask, B, N, n, m, dtype, stream, nullptr);
 }
 
 int socu_adjoint_scatter(const void* bbar, const void* x, void* dL, void* dE, const unsigned char* mask, int B, int N,
@@ -301,7 +357,7 @@ int socu_adjoint_scatter(const void* bbar, const void* x, void* dL, void* dE, co
 }
 
 int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
-                    int dtype, void* stream, double* kernel_ms, int* kernel_launches) {
+                    int dtype, void* work, void* stream, double* kernel_ms, int* kernel_launches) {
   if (kernel_ms == nullptr || kernel_launches == nullptr) return fail(SOCU_EINVAL, "output arrays are NULL");
   for (int c = 0; c < socu::kKcCount; ++c) {
     kernel_ms[c] = 0.0;
@@ -309,7 +365,7 @@ int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask
   }
   g_timer = TimerState();
   g_timer.active = true;
-  int rc = dispatch(op, L, E, x, mask, B, N, n, m, dtype, stream);
+  int rc = dispatch(op, L, E, x, mask, B, N, n, m, dtype, stream, work);
   g_timer.active = false;
   hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
   for (size_t q = 0; q < g_timer.cls.size(); ++q) {
@@ -340,8 +396,15 @@ int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const u
     if (rc) return rc;
   }
   socu_plan* p = new socu_plan{op, L, E, x, mask, B, N, n, need_x ? m : 1, dtype, nullptr, nullptr, SweepWs(),
-                               socu::SweepOverlap(), false};
+                               socu::SweepOverlap(), false, nullptr};
   rc = alloc_sweep_ws(p->sweep, op, B, N, n, p->m, dtype);
+  if (rc == SOCU_OK && op_needs_inverse(op) && B > 0) {
+    const hipError_t e = hipMalloc(&p->inv, inverse_ws_bytes(B, N, n, dtype));
+    if (e != hipSuccess) {
+      p->inv = nullptr;
+      rc = hip_fail(e, "hipMalloc (diagonal-block inverse workspace)");
+    }
+  }
   // The forward sweep alongside the factor pays inside a graph only: issued
   // eagerly, its event records and waits cost more host time than the
   // overlap saves (n=128 N=512 FP64: 1.19 ms without, 1.24 ms with).
@@ -367,7 +430,8 @@ int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const u
       socu_plan_destroy(p);  // workspace and streams; no graph yet
       return hip_fail(e, "hipStreamBeginCapture");
     }
-    rc = dispatch(op, L, E, x, mask, B, N, n, p->m, dtype, cs, p->sweep, p->has_overlap ? &p->overlap : nullptr);
+    rc = dispatch(op, L, E, x, mask, B, N, n, p->m, dtype, cs, p->inv, p->sweep,
+                  p->has_overlap ? &p->overlap : nullptr);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
@@ -393,18 +457,20 @@ int socu_plan_create(socu_plan** out, int op, void* L, void* E, void* x, const u
 int socu_plan_execute(socu_plan* p, void* stream) {
   if (p == nullptr) return fail(SOCU_EINVAL, "plan is NULL");
   if (p->B == 0) return SOCU_OK;
+  const int rc = check_expired(p->sweep.hstatus);
+  if (rc) return rc;
   if (p->exec) {
     hipError_t e = hipGraphLaunch(p->exec, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
     return SOCU_OK;
   }
-  return dispatch(p->op, p->L, p->E, p->x, p->mask, p->B, p->N, p->n, p->m, p->dtype, stream, p->sweep,
+  return dispatch(p->op, p->L, p->E, p->x, p->mask, p->B, p->N, p->n, p->m, p->dtype, stream, p->inv, p->sweep,
                   p->has_overlap ? &p->overlap : nullptr);
 }
 
 int socu_plan_sweep_status(socu_plan* p, int* persistent, int* expired, void* stream) {
   if (p == nullptr) return fail(SOCU_EINVAL, "plan is NULL");
-  return sweep_status(p->sweep.ws, persistent, expired, stream);
+  return sweep_status(p->sweep.ws, p->sweep.hstatus, persistent, expired, stream);
 }
 
 int socu_plan_destroy(socu_plan* p) {
@@ -412,6 +478,7 @@ int socu_plan_destroy(socu_plan* p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   free_sweep_ws(p->sweep);
+  if (p->inv) (void)hipFree(p->inv);
   if (p->has_overlap) socu::overlap_destroy(p->overlap);
   delete p;
   return SOCU_OK;
@@ -492,6 +559,8 @@ int socu_mixed_create(socu_mixed** out, const void* L, const void* E, void* x, c
 int socu_mixed_execute(socu_mixed* h, void* stream) {
   if (h == nullptr) return fail(SOCU_EINVAL, "handle is NULL");
   if (h->p.B == 0) return SOCU_OK;
+  const int rc = check_expired(h->p.sweep_hstatus);
+  if (rc) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = h->exec ? hipGraphLaunch(h->exec, st) : socu::mixed_enqueue(h->p, st);
   if (e != hipSuccess) return hip_fail(e, "mixed-precision solve launch");
@@ -543,7 +612,7 @@ int socu_mixed_profile(socu_mixed* h, void* stream, double* kernel_ms, int* kern
 
 int socu_mixed_sweep_status(socu_mixed* h, int* persistent, int* expired, void* stream) {
   if (h == nullptr) return fail(SOCU_EINVAL, "handle is NULL");
-  return sweep_status(h->p.sweep_ws, persistent, expired, stream);
+  return sweep_status(h->p.sweep_ws, h->p.sweep_hstatus, persistent, expired, stream);
 }
 
 int socu_mixed_destroy(socu_mixed* h) {

@@ -274,6 +274,91 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
 }
 
 // ---------------------------------------------------------------------------
+// Inverses of the diagonal factor blocks from tril(L) alone, for solves whose
+// factor may come from anywhere (the reference's sweeps read only tril(L),
+// socu/blas/potrf.py:101-103).  One workgroup of NB wavefronts per block
+// (grid (N, B)): W_i <- tril(L_i) with the transposed inverse in its strict
+// upper triangle, W_i[k][r] = Linv_i[r][k] for k < r -- the layout k_potrf_blk
+// leaves in L_i, which the sweep and apply kernels read.  L is only read.
+//   wave w inverts diagonal tile w (tile_tri_inv);
+//   band d = 1..NB-1: Linv_ij = -Dinv_i sum_{k=j}^{i-1} L_ik Linv_kj, i = j+d,
+//   one wavefront per tile, MFMA.
+// ---------------------------------------------------------------------------
+template <typename T, int NB>
+constexpr size_t trtri_blk_lds() {
+  return sizeof(T) * ((size_t)(32 * NB) * (32 * NB + 1) + 32 * NB);
+}
+
+template <typename T, int NB, int NF>
+__global__ __launch_bounds__(64 * NB) void k_trtri_blk(const T* __restrict__ L, T* __restrict__ W,
+                                                      const uint8_t* __restrict__ mask, int N, int n_) {
+  const int n = NF ? NF : n_;
+  constexpr int NP = 32 * NB;
+  constexpr int P = NP + 1;
+  constexpr int TH = 64 * NB;
+  const int b = blockIdx.y;
+  if (masked_out(mask, b)) return;
+  const int i = blockIdx.x;
+  const size_t nn = (size_t)n * n;
+  const T* A = L + ((size_t)b * N + i) * nn;
+  T* Wb = W + ((size_t)b * N + i) * nn;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* S = reinterpret_cast<T*>(smem_raw);
+  T* dinv = S + NP * P;
+  const int tid = threadIdx.x, wave = wave_id();
+
+  // tril(L_i) into the padded square: zeros above the diagonal and outside
+  // n, ones on the padded part of the diagonal; the block is read as one
+  // contiguous run (upper entries are loaded and dropped).
+  for (int idx = tid; idx < NP * NP; idx += TH) {
+    const int r = idx / NP, c = idx - r * NP;
+    T v = (r == c) ? T(1) : T(0);
+    if (r < n && c < n) v = (c <= r) ? A[r * n + c] : T(0);
+    S[r * P + c] = v;
+  }
+  __syncthreads();
+  tile_tri_inv<T>(S + 32 * wave * P + 32 * wave, P, dinv + 32 * wave);
+  __syncthreads();
+  for (int d = 1; d < NB; ++d) {
+    const int j = wave, ii = wave + d;
+    if (ii < NB) {
+      // S_ij -> upper tile (j, ii), transposed
+      typename Mfma<T>::acc_t acc[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+      for (int k = j; k < ii; ++k) {
+        const T* Lik = S + 32 * ii * P + 32 * k;
+        const T* Ukj = S + 32 * j * P + 32 * k;  // Linv_kj transposed (k > j), or tile j's own inverse
+        const T* dj = dinv + 32 * k;
+        wave_gemm_f<T>(
+            acc, [&](int r, int kk) { return Lik[r * P + kk]; },
+            [&](int kk, int c) {
+              if (k != j) return Ukj[c * P + kk];
+              return (kk > c) ? Ukj[c * P + kk] : ((kk == c) ? dj[c] : T(0));
+            },
+            32);
+      }
+      T* Uji = S + 32 * j * P + 32 * ii;
+      acc_foreach<T>(acc, [&](int r, int c, T v) { Uji[c * P + r] = v; });
+      // Linv_ij = -Dinv_i S_ij, in place.  Only this wavefront touches tile
+      // (j, ii), and the LDS operations of one wavefront complete in issue
+      // order, so its stores above precede these reads and these reads the
+      // stores below.
+      const T* Di = S + 32 * ii * P + 32 * ii;
+      const T* di = dinv + 32 * ii;
+      typename Mfma<T>::acc_t inv[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+      wave_gemm_f<T>(
+          inv, [&](int r, int kk) { return (r > kk) ? Di[kk * P + r] : ((r == kk) ? di[r] : T(0)); },
+          [&](int kk, int c) { return Uji[c * P + kk]; }, 32);
+      acc_foreach<T>(inv, [&](int r, int c, T v) { Uji[c * P + r] = -v; });
+    }
+    __syncthreads();  // band d is final before band d + 1 reads it
+  }
+  for (int idx = tid; idx < (int)nn; idx += TH) {
+    const int r = idx / n, c = idx - r * n;
+    Wb[idx] = S[r * P + c];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // trsm with the explicit inverse left by k_potrf_blk in the strict upper
 // triangle of L_i (U[k][r] = Linv[r][k] for k < r, 1/L[r][r] on the diagonal):
 //   R task  (32 rows of R):   R  <- R Linv^T,   wave q: output columns 32q..
@@ -1219,6 +1304,15 @@ static bool use_schur_small(int nodes, int n, int esize) {
   return (double)nodes * n * n * esize <= 12.0 * 1024 * 1024;
 }
 
+template <typename T, int NB>
+static hipError_t set_trtri_attrs() {
+  hipError_t e = hipFuncSetAttribute((const void*)k_trtri_blk<T, NB, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)trtri_blk_lds<T, NB>());
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_trtri_blk<T, NB, 32 * NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)trtri_blk_lds<T, NB>());
+}
+
 template <int NB>
 static hipError_t set_mixed_attrs() {
   hipError_t e = hipFuncSetAttribute((const void*)k_potrf_blk<float, NB, 0, double>,
@@ -1246,6 +1340,14 @@ hipError_t init_blocked_kernels() {
   if ((e = set_schur_small_attrs<double, 2>()) != hipSuccess) return e;
   if ((e = set_schur_small_attrs<double, 3>()) != hipSuccess) return e;
   if ((e = set_schur_small_attrs<double, 4>()) != hipSuccess) return e;
+  if ((e = set_trtri_attrs<float, 1>()) != hipSuccess) return e;
+  if ((e = set_trtri_attrs<float, 2>()) != hipSuccess) return e;
+  if ((e = set_trtri_attrs<float, 3>()) != hipSuccess) return e;
+  if ((e = set_trtri_attrs<float, 4>()) != hipSuccess) return e;
+  if ((e = set_trtri_attrs<double, 1>()) != hipSuccess) return e;
+  if ((e = set_trtri_attrs<double, 2>()) != hipSuccess) return e;
+  if ((e = set_trtri_attrs<double, 3>()) != hipSuccess) return e;
+  if ((e = set_trtri_attrs<double, 4>()) != hipSuccess) return e;
   if ((e = set_mixed_attrs<1>()) != hipSuccess) return e;
   if ((e = set_mixed_attrs<2>()) != hipSuccess) return e;
   if ((e = set_mixed_attrs<3>()) != hipSuccess) return e;
@@ -1338,6 +1440,34 @@ hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, i
     case 2: return factor_level_nb<T, 2, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
     case 3: return factor_level_nb<T, 3, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
     case 4: return factor_level_nb<T, 4, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T, int NB, int NF>
+static hipError_t trtri_nb(const T* L, T* W, const uint8_t* mask, int B, int N, int n, hipStream_t st) {
+  if (B <= 0) return hipSuccess;
+  const size_t lds = trtri_blk_lds<T, NB>();
+  kt_begin(kKcForward, st);
+  hipLaunchKernelGGL((k_trtri_blk<T, NB, NF>), dim3(N, B), dim3(64 * NB), lds, st, L, W, mask, N, n);
+  kt_end(kKcForward, st);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t trtri_blocked(const T* L, T* W, const uint8_t* mask, int B, int N, int n, hipStream_t st) {
+  switch (n) {
+    case 32: return trtri_nb<T, 1, 32>(L, W, mask, B, N, n, st);
+    case 64: return trtri_nb<T, 2, 64>(L, W, mask, B, N, n, st);
+    case 96: return trtri_nb<T, 3, 96>(L, W, mask, B, N, n, st);
+    case 128: return trtri_nb<T, 4, 128>(L, W, mask, B, N, n, st);
+    default: break;
+  }
+  switch ((n + 31) / 32) {
+    case 1: return trtri_nb<T, 1, 0>(L, W, mask, B, N, n, st);
+    case 2: return trtri_nb<T, 2, 0>(L, W, mask, B, N, n, st);
+    case 3: return trtri_nb<T, 3, 0>(L, W, mask, B, N, n, st);
+    case 4: return trtri_nb<T, 4, 0>(L, W, mask, B, N, n, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1499,6 +1629,8 @@ template hipError_t factor_with_forward<double>(double*, double*, double*, const
                                                 int, int, bool, const SweepOverlap&, hipStream_t,
                                                 const std::function<hipError_t(hipStream_t)>*);
 
+template hipError_t trtri_blocked<float>(const float*, float*, const uint8_t*, int, int, int, hipStream_t);
+template hipError_t trtri_blocked<double>(const double*, double*, const uint8_t*, int, int, int, hipStream_t);
 template hipError_t factor_level_blocked<float>(float*, float*, const uint8_t*, int, int, int, int, bool,
                                                 hipStream_t, hipEvent_t, hipEvent_t, hipEvent_t);
 template hipError_t factor_level_blocked<double>(double*, double*, const uint8_t*, int, int, int, int, bool,

@@ -42,6 +42,13 @@ hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, i
                                 hipStream_t st, hipEvent_t after_potrf = nullptr, hipEvent_t after_trsm = nullptr,
                                 hipEvent_t before_trsm = nullptr);
 
+// W <- tril(L) with the transposed inverse of each diagonal block in its
+// strict upper triangle (the layout the factor leaves in L), from tril(L)
+// only; one launch over the B*N blocks.  Solves of a factor that need not
+// come from this library's factor read W in place of L.
+template <typename T>
+hipError_t trtri_blocked(const T* L, T* W, const uint8_t* mask, int B, int N, int n, hipStream_t st);
+
 // Forward-substitution step: step -1 applies level 0; step l >= 0 applies
 // level l's updates to every receiving block, then (unless apply_next is
 // false) level l+1's inverses.
@@ -84,9 +91,16 @@ size_t sweep_slot_elems(int B, int N, int n);
 // Forward and/or backward substitution of x (B, N, n, 1) with the factor in
 // L, E; one clear launch (skipped with clear = false, for callers that zero
 // the workspace themselves) and one persistent launch.
+// hstatus (optional): a host-visible word (pinned host memory) that an
+// expired wait sets to 1, so the host can see it without synchronising.
 template <typename T>
 hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, bool fwd, bool bwd,
-                            int* ws, T* slots, hipStream_t st, bool clear = true);
+                            int* ws, T* slots, int* hstatus, hipStream_t st, bool clear = true);
+// Polls per dependency wait for kernels enqueued from now on; 0 restores the
+// default (kSweepSpinLimit), a negative value makes every wait that is not
+// already satisfied expire at once (for tests of the failure path).
+void set_sweep_spin_limit(int polls);
+int sweep_spin_limit();
 // Whether a plan for block size n and element size dsize uses the persistent
 // sweeps (plans with one right-hand side only): yes, they measured faster than
 // the level kernels for every block size and precision, unless the
@@ -136,6 +150,8 @@ struct MixedPlan {
   int nblk_init = 1;
   int* sweep_ws = nullptr;     // persistent sweeps (m == 1), else NULL
   float* sweep_slots = nullptr;
+  int* sweep_hstatus = nullptr;  // pinned host word raised by an expired wait
+  int* sweep_hstatus_dev = nullptr;  // its device address
   double* res_w = nullptr;     // one-RHS residual pieces (m == 1), else NULL
   double* res_v = nullptr;
   double* res_c = nullptr;

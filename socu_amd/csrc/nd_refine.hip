@@ -549,6 +549,12 @@ hipError_t mixed_alloc(MixedPlan& p) {
     if ((e = hipMalloc(&p.sweep_ws, sizeof(int) * sweep_ws_ints(p.B, p.N))) != hipSuccess) return e;
     if ((e = hipMemset(p.sweep_ws, 0, sizeof(int) * sweep_ws_ints(p.B, p.N))) != hipSuccess) return e;
     if ((e = hipMalloc(&p.sweep_slots, sizeof(float) * sweep_slot_elems(p.B, p.N, p.n))) != hipSuccess) return e;
+    if ((e = hipHostMalloc(&p.sweep_hstatus, sizeof(int), hipHostMallocMapped)) != hipSuccess) {
+      p.sweep_hstatus = nullptr;
+      return e;
+    }
+    *p.sweep_hstatus = 0;
+    if ((e = hipHostGetDevicePointer((void**)&p.sweep_hstatus_dev, p.sweep_hstatus, 0)) != hipSuccess) return e;
   }
   return p.use_overlap ? overlap_create(p.overlap) : hipSuccess;
 }
@@ -558,6 +564,8 @@ void mixed_free(MixedPlan& p) {
                   p.sweep_ws, p.sweep_slots, p.res_w, p.res_v, p.res_c, p.res_ticket};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  if (p.sweep_hstatus) (void)hipHostFree(p.sweep_hstatus);
+  p.sweep_hstatus = p.sweep_hstatus_dev = nullptr;
   p.L32 = p.E32 = p.d32 = nullptr;
   p.b64 = p.state = p.partial = nullptr;
   p.active = nullptr;
@@ -633,7 +641,7 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
       // (the workspace is re-armed by k_refine_start and by k_resid1_finish
       // or k_refine_check)
       if ((e = sweep_persistent<float>(p.L32, p.E32, p.d32, p.active, B, N, n, fwd, true, p.sweep_ws, p.sweep_slots,
-                                       st, false)) != hipSuccess)
+                                       p.sweep_hstatus_dev, st, false)) != hipSuccess)
         return e;
     } else {
       if (fwd)

@@ -37,6 +37,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
+
 #include "nd_common.h"
 #include "nd_internal.h"
 #include "nd_tile.h"
@@ -211,14 +213,19 @@ __device__ __forceinline__ bool sweep_decode(int task, int B, int N, int Lv, int
 // no release fence (which writes back the whole L2: 1 us for a lone task,
 // 7-9 us per task with 32-64 tasks in flight) and no acquire invalidation.
 
-// Bounded wait (one thread): until *p >= target.  Returns false, and raises
-// the status word, when the bound expires or an earlier wait already expired.
-__device__ bool wait_at_least(const int* p, int target, int* status) {
+// Bounded wait (one thread): until *p >= target, at most `limit` polls.
+// Returns false, and raises the status word (and the host-visible word
+// hstatus, if any), when the bound expires or an earlier wait already
+// expired.  limit < 0 makes every wait that is not already satisfied expire
+// at once (a host-side switch for tests of this path; every kernel still
+// exits normally).
+__device__ bool wait_at_least(const int* p, int target, int* status, int* hstatus, int limit) {
   int spins = 0;
   while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
     if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
-    if (++spins > kSweepSpinLimit) {
+    if (limit < 0 || ++spins > limit) {
       __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (hstatus != nullptr) __hip_atomic_store(hstatus, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       return false;
     }
     __builtin_amdgcn_s_sleep(1);
@@ -278,7 +285,8 @@ constexpr size_t sweep_lds_bytes() {
 template <typename T, int NB>
 __global__ __launch_bounds__((SweepShape<T, NB>::THREADS)) void k_sweep(const T* __restrict__ L, const T* __restrict__ E,
                                                                    T* x, const uint8_t* __restrict__ mask, int* ws,
-                                                                   T* slots, int B, int N, int n, int dirs, int lmin) {
+                                                                   T* slots, int* hstatus, int spin_limit, int B,
+                                                                   int N, int n, int dirs, int lmin) {
   using S = SweepShape<T, NB>;
   constexpr int NP = S::NP;
   // Three blocks in flight at once when they fit in registers; otherwise the
@@ -299,7 +307,7 @@ __global__ __launch_bounds__((SweepShape<T, NB>::THREADS)) void k_sweep(const T*
   T* Us = reinterpret_cast<T*>(sweep_smem);
   __shared__ T Y[NP], Y2[NP], XA[NP], XC[NP], D[NP], RW[NP];
   __shared__ T part[S::H * NP], part2[S::H * NP];
-  __shared__ int s_task;
+  __shared__ int s_task, s_ok;
   const int t = threadIdx.x;
 
   for (;;) {
@@ -364,15 +372,19 @@ __global__ __launch_bounds__((SweepShape<T, NB>::THREADS)) void k_sweep(const T*
         if (!bwd) {
           int need = 0;
           for (int lp = lmin; lp < l; ++lp) need += 1 + ((j + (1 << lp) < N) ? 1 : 0);
-          if (need > 0) ok = wait_at_least(fcount + (size_t)b * N + j, need, status);
+          if (need > 0) ok = wait_at_least(fcount + (size_t)b * N + j, need, status, hstatus, spin_limit);
         } else {
-          if (fwd) ok = wait_at_least(fdone + (size_t)b * N + j, 1, status);
-          if (ok && hasR) ok = wait_at_least(bdone + (size_t)b * N + j + s, 1, status);
-          if (ok && hasC) ok = wait_at_least(bdone + (size_t)b * N + j - s, 1, status);
+          if (fwd) ok = wait_at_least(fdone + (size_t)b * N + j, 1, status, hstatus, spin_limit);
+          if (ok && hasR) ok = wait_at_least(bdone + (size_t)b * N + j + s, 1, status, hstatus, spin_limit);
+          if (ok && hasC) ok = wait_at_least(bdone + (size_t)b * N + j - s, 1, status, hstatus, spin_limit);
         }
-        (void)ok;
+        s_ok = ok ? 1 : 0;
       }
       __syncthreads();
+      // After an expired wait this task's inputs may be incomplete: its
+      // output is NaN (and so is everything computed from it), so the failure
+      // shows in the solution, besides the status words.
+      const bool ok = s_ok != 0;
 
       if (!bwd) {
         // y_j = b_j - contributions, summed in a fixed order
@@ -409,6 +421,7 @@ __global__ __launch_bounds__((SweepShape<T, NB>::THREADS)) void k_sweep(const T*
             v = Y[t] / D[t];
 #pragma unroll
             for (int h = 0; h < S::H; ++h) v += part[h * NP + t];
+            if (!ok) v = __builtin_nan("");
             st_coh(xj + t, v);
           }
           Y2[t] = v;
@@ -474,6 +487,7 @@ __global__ __launch_bounds__((SweepShape<T, NB>::THREADS)) void k_sweep(const T*
           T v = Y2[t] / D[t];
 #pragma unroll
           for (int h = 0; h < S::H; ++h) v += part2[h * NP + t];
+          if (!ok) v = __builtin_nan("");
           st_coh(xj + t, v);
         }
         stores_done();
@@ -489,6 +503,16 @@ __global__ __launch_bounds__((SweepShape<T, NB>::THREADS)) void k_sweep(const T*
 __global__ __launch_bounds__(kThreads) void k_sweep_clear(int* __restrict__ ws, int count) {
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < count; i += gridDim.x * kThreads) ws[i] = 0;
 }
+
+}  // namespace
+
+// Polls per dependency wait of the kernels enqueued from now on (a graph
+// keeps the value it was captured with).
+static std::atomic<int> g_spin_limit{kSweepSpinLimit};
+void set_sweep_spin_limit(int polls) { g_spin_limit.store(polls == 0 ? kSweepSpinLimit : polls); }
+int sweep_spin_limit() { return g_spin_limit.load(); }
+
+namespace {
 
 // Resident workgroups per CU of each instantiation, and the CU count.
 int g_sweep_occ[2][4];
@@ -506,12 +530,12 @@ hipError_t sweep_occupancy(int& out) {
 }
 
 template <typename T, int NB>
-hipError_t launch_sweep(const T* L, const T* E, T* x, const uint8_t* mask, int* ws, T* slots, int B, int N, int n,
-                        int dirs, int lmin, int grid, hipStream_t st) {
+hipError_t launch_sweep(const T* L, const T* E, T* x, const uint8_t* mask, int* ws, T* slots, int* hstatus, int B,
+                        int N, int n, int dirs, int lmin, int grid, hipStream_t st) {
   const size_t lds = sweep_lds_bytes<T, NB>();
   constexpr int threads = SweepShape<T, NB>::THREADS;
-  hipLaunchKernelGGL((k_sweep<T, NB>), dim3(grid), dim3(threads), lds, st, L, E, x, mask, ws, slots,
-                     B, N, n, dirs, lmin);
+  hipLaunchKernelGGL((k_sweep<T, NB>), dim3(grid), dim3(threads), lds, st, L, E, x, mask, ws, slots, hstatus,
+                     sweep_spin_limit(), B, N, n, dirs, lmin);
   return hipGetLastError();
 }
 
@@ -572,7 +596,7 @@ int sweep_lmin(int B, int N, int n, int dsize) {
 
 template <typename T>
 hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, bool fwd, bool bwd,
-                            int* ws, T* slots, hipStream_t st, bool clear) {
+                            int* ws, T* slots, int* hstatus, hipStream_t st, bool clear) {
   if (B <= 0 || (!fwd && !bwd)) return hipSuccess;
   const int Lv = recursive_iterations(N);
   const int lmin = sweep_lmin(B, N, n, (int)sizeof(T));
@@ -599,10 +623,10 @@ hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, i
   if (clear)
     hipLaunchKernelGGL(k_sweep_clear, dim3(cblocks < 1024 ? cblocks : 1024), dim3(kThreads), 0, st, ws, nclear);
   switch (nb) {
-    case 1: e = launch_sweep<T, 1>(L, E, x, mask, ws, slots, B, N, n, dirs, lmin, grid, st); break;
-    case 2: e = launch_sweep<T, 2>(L, E, x, mask, ws, slots, B, N, n, dirs, lmin, grid, st); break;
-    case 3: e = launch_sweep<T, 3>(L, E, x, mask, ws, slots, B, N, n, dirs, lmin, grid, st); break;
-    case 4: e = launch_sweep<T, 4>(L, E, x, mask, ws, slots, B, N, n, dirs, lmin, grid, st); break;
+    case 1: e = launch_sweep<T, 1>(L, E, x, mask, ws, slots, hstatus, B, N, n, dirs, lmin, grid, st); break;
+    case 2: e = launch_sweep<T, 2>(L, E, x, mask, ws, slots, hstatus, B, N, n, dirs, lmin, grid, st); break;
+    case 3: e = launch_sweep<T, 3>(L, E, x, mask, ws, slots, hstatus, B, N, n, dirs, lmin, grid, st); break;
+    case 4: e = launch_sweep<T, 4>(L, E, x, mask, ws, slots, hstatus, B, N, n, dirs, lmin, grid, st); break;
     default: e = hipErrorInvalidValue; break;
   }
   kt_end(fwd ? kKcForward : kKcBackward, st);
@@ -614,8 +638,8 @@ hipError_t sweep_persistent(const T* L, const T* E, T* x, const uint8_t* mask, i
 }
 
 template hipError_t sweep_persistent<float>(const float*, const float*, float*, const uint8_t*, int, int, int, bool,
-                                            bool, int*, float*, hipStream_t, bool);
+                                            bool, int*, float*, int*, hipStream_t, bool);
 template hipError_t sweep_persistent<double>(const double*, const double*, double*, const uint8_t*, int, int, int,
-                                             bool, bool, int*, double*, hipStream_t, bool);
+                                             bool, bool, int*, double*, int*, hipStream_t, bool);
 
 }  // namespace socu

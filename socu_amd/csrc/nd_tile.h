@@ -303,6 +303,42 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col, 
     for (int c = 0; c < 32; ++c) ut[r * 33 + c] = u[c];
 }
 
+// Inverse of a 32x32 lower-triangular tile (no factorisation), by one
+// wavefront: lane c (of 0-31) forms column c of the inverse by forward
+// substitution, x_r = (delta_rc - sum_{k<r} D[r][k] x_k) / D[r][r], with
+// rows of D read as LDS broadcasts and four partial sums per row; lanes 32-63
+// repeat lanes 0-31 and store nothing.  Output convention of tile_chol_inv:
+// the strict upper triangle receives the transposed inverse
+// (S[c][q] = Dinv[q][c] for q > c), dinv[r] = 1/D[r][r]; the lower triangle
+// (the input) is unchanged.
+template <typename T>
+__device__ __forceinline__ void tile_tri_inv(T* S, int pitch, T* dinv) {
+  const int lane = lane_id();
+  const int c = lane & 31;
+  T x[32];
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+#pragma unroll
+    for (int k = 0; k + 3 < r; k += 4) {
+      s0 = fma(S[r * pitch + k], x[k], s0);
+      s1 = fma(S[r * pitch + k + 1], x[k + 1], s1);
+      s2 = fma(S[r * pitch + k + 2], x[k + 2], s2);
+      s3 = fma(S[r * pitch + k + 3], x[k + 3], s3);
+    }
+#pragma unroll
+    for (int k = r & ~3; k < r; ++k) s0 = fma(S[r * pitch + k], x[k], s0);
+    const T dr = T(1) / S[r * pitch + r];
+    x[r] = (((r == c) ? T(1) : T(0)) - ((s0 + s1) + (s2 + s3))) * dr;
+  }
+  if (lane < 32) {
+#pragma unroll
+    for (int q = 0; q < 32; ++q)
+      if (q > c) S[c * pitch + q] = x[q];
+    dinv[c] = x[c];
+  }
+}
+
 // Cholesky and inverse of a 16x16 tile by lanes 0-31 of the wavefront (row
 // lanes 0-15, inverse lanes 16-31; lanes 32-63 must not call it), with the
 // recurrence and in/out convention of tile_chol_inv.

@@ -111,6 +111,12 @@ def nd_flops(N: int, n: int, m: int):
     return f["potrf"] + f["trsm"] + f["schur"], f["forward"] + f["backward"]
 
 
+def block_matvec_flops(N: int, n: int, m: int):
+    """Flops of Psi x for one system: a dense n x n block times n x m per
+    diagonal block and two per coupling (E_i and E_i^T)."""
+    return 2 * n * n * m * (N + 2 * max(N - 1, 0))
+
+
 def nd_bytes(N: int, n: int, m: int, w: int = 8):
     """Compulsory HBM bytes (factor, solve, factor_and_solve) with w-byte
     scalars: the factor reads and writes tril L, reads the N-1 input E blocks

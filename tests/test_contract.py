@@ -1,0 +1,216 @@
+"""Drop-in contract of the substitution launches and the failure path of the
+persistent sweeps.
+
+* The reference's solve, forward and backward substitution read only tril(L)
+  and E (socu/blas/potrf.py:101-103 "Only the lower triangular part is
+  used"): a factor made anywhere -- here by the oracle -- with arbitrary
+  values in the strict upper triangle of each L[i] must give the oracle's y
+  and x (FP64 to 1e-11 relative), and the launches must leave L and E
+  bit-identical.
+* A bounded dependency wait of the persistent sweeps that expires must show:
+  the solution holds NaN and the next call of the launch raises.  The waits
+  are forced to expire from the host (socu_set_sweep_spin_limit(-1): every
+  wait that is not already satisfied gives up at once), so every kernel
+  still exits normally.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import nd
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
+
+
+def _stack(n, N, m, B, seed, alpha=16.0):
+    Ls, Es, bs = [], [], []
+    for k in range(B):
+        L0, E0, b0, _, _ = nd.readme_problem(n, N, m, alpha=alpha, seed=seed + k)
+        Ls.append(L0)
+        Es.append(E0)
+        bs.append(b0)
+    return np.stack(Ls), np.stack(Es), np.stack(bs)
+
+
+@pytest.mark.parametrize("n,N,m", [(8, 20, 1), (32, 33, 1), (36, 17, 3), (64, 40, 1), (100, 9, 2), (128, 24, 1),
+                                   (128, 16, 17)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_substitutions_of_a_foreign_factor(n, N, m, graph, device):
+    import torch
+    from socu_amd.block_tridiag_solver import (
+        create_cholesky_backward_substitution_launch,
+        create_cholesky_forward_substitution_launch,
+        create_cholesky_solve_launch,
+    )
+
+    B = 2
+    L0, E0, b0 = _stack(n, N, m, B, seed=40)
+    Lf, Ef = L0.copy(), E0.copy()
+    nd.factor(Lf, Ef)
+    yr = b0.copy()
+    nd.forward(Lf, Ef, yr)
+    xr = yr.copy()
+    nd.backward(Lf, Ef, xr)
+    # the oracle's factor in the lower triangle, noise above it
+    rs = np.random.RandomState(5)
+    noise = np.triu(rs.randn(*Lf.shape) * 1e3, 1)
+    Lup = np.tril(Lf) + noise
+    L = torch.tensor(Lup, device=device)
+    E = torch.tensor(Ef, device=device)
+
+    x = torch.tensor(b0, device=device)
+    create_cholesky_forward_substitution_launch(L, E, x, use_cuda_graph=graph)()
+    assert _rel(x.cpu().numpy(), yr) < 1e-11
+    create_cholesky_backward_substitution_launch(L, E, x, use_cuda_graph=graph)()
+    assert _rel(x.cpu().numpy(), xr) < 1e-11
+    x.copy_(torch.tensor(b0, device=device))
+    solve = create_cholesky_solve_launch(L, E, x, use_cuda_graph=graph)
+    solve()
+    assert _rel(x.cpu().numpy(), xr) < 1e-11
+    # a second call on a fresh right-hand side (graph replay)
+    x.copy_(torch.tensor(b0, device=device))
+    solve()
+    assert _rel(x.cpu().numpy(), xr) < 1e-11
+    np.testing.assert_array_equal(L.cpu().numpy(), Lup)
+    np.testing.assert_array_equal(E.cpu().numpy(), Ef)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_solve_after_factor_of_this_library_then_tril(dtype, device):
+    """factor with this library, clear the strict upper triangle (L.tril_()),
+    then solve: the result does not depend on what the factor left above the
+    diagonal."""
+    import torch
+    from socu_amd.block_tridiag_solver import create_cholesky_factor_launch, create_cholesky_solve_launch
+
+    tdt = getattr(torch, dtype)
+    n, N, m = 64, 33, 1
+    L0, E0, b0 = _stack(n, N, m, 1, seed=3)
+    L = torch.tensor(L0, dtype=tdt, device=device)
+    E = torch.tensor(E0, dtype=tdt, device=device)
+    create_cholesky_factor_launch(L, E, dtype=tdt)()
+    x1 = torch.tensor(b0, dtype=tdt, device=device)
+    create_cholesky_solve_launch(L, E, x1, dtype=tdt)()
+    L.tril_()
+    x2 = torch.tensor(b0, dtype=tdt, device=device)
+    create_cholesky_solve_launch(L, E, x2, dtype=tdt)()
+    np.testing.assert_array_equal(x1.cpu().numpy(), x2.cpu().numpy())
+
+
+class _SpinLimit:
+    def __init__(self, polls):
+        self.polls = polls
+
+    def __enter__(self):
+        from socu_amd import _lib
+
+        _lib.load().socu_set_sweep_spin_limit(self.polls)
+
+    def __exit__(self, *exc):
+        from socu_amd import _lib
+
+        _lib.load().socu_set_sweep_spin_limit(0)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_expired_wait_gives_nan_and_the_next_call_raises(graph, device):
+    import torch
+    from socu_amd import _lib
+    from socu_amd.block_tridiag_solver import create_cholesky_factor_and_solve_launch
+
+    n, N = 32, 64
+    L0, E0, b0 = _stack(n, N, 1, 1, seed=8)
+    L = torch.tensor(L0, device=device)
+    E = torch.tensor(E0, device=device)
+    x = torch.tensor(b0, device=device)
+    with _SpinLimit(-1):
+        run = create_cholesky_factor_and_solve_launch(L, E, x, use_cuda_graph=graph)
+        run()
+        torch.cuda.synchronize()
+    assert np.isnan(x.cpu().numpy()).any()
+    with pytest.raises(_lib.SweepExpiredError):
+        run()
+    # the word is cleared once reported; a launch made with the default bound
+    # is unaffected
+    L.copy_(torch.tensor(L0, device=device))
+    E.copy_(torch.tensor(E0, device=device))
+    x.copy_(torch.tensor(b0, device=device))
+    ok = create_cholesky_factor_and_solve_launch(L, E, x, use_cuda_graph=graph)
+    ok()
+    assert ok.sweep_status() == (True, False)
+    assert nd.rel_residual(L0, E0, x.cpu().numpy(), b0) < 1e-12
+
+
+def test_expired_wait_reported_by_sweep_status(device):
+    import torch
+    from socu_amd.block_tridiag_solver import create_cholesky_solve_launch
+
+    n, N = 16, 40
+    L0, E0, b0 = _stack(n, N, 1, 1, seed=9)
+    Lf, Ef = L0.copy(), E0.copy()
+    nd.factor(Lf, Ef)
+    L = torch.tensor(Lf, device=device)
+    E = torch.tensor(Ef, device=device)
+    x = torch.tensor(b0, device=device)
+    with _SpinLimit(-1):
+        run = create_cholesky_solve_launch(L, E, x)
+        run()
+    assert run.sweep_status() == (True, True)
+    assert np.isnan(x.cpu().numpy()).any()
+    # reported by sweep_status, so the next call runs (with the limit it was
+    # enqueued under restored: the default)
+    x.copy_(torch.tensor(b0, device=device))
+    run()
+    assert run.sweep_status() == (True, False)
+    xr = b0.copy()
+    nd.solve(Lf, Ef, xr)
+    assert _rel(x.cpu().numpy(), xr) < 1e-11
+
+
+def test_mixed_expired_wait_raises_next_call(device):
+    import torch
+    from socu_amd import _lib
+    from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
+
+    n, N = 32, 64
+    L0, E0, b0 = _stack(n, N, 1, 1, seed=10)
+    x = torch.tensor(b0, device=device)
+    with _SpinLimit(-1):
+        run = create_cholesky_mixed_factor_and_solve_launch(torch.tensor(L0, device=device),
+                                                            torch.tensor(E0, device=device), x, max_iter=3,
+                                                            use_cuda_graph=True)
+        run()
+        torch.cuda.synchronize()
+    rr, _ = run.stats()
+    assert not (rr[0] <= 1e-10)
+    with pytest.raises(_lib.SweepExpiredError):
+        run()
+
+
+def test_direct_solve_entry_point_with_workspace(device):
+    """socu_cholesky_solve through the C-ABI with a caller-provided workspace,
+    on a foreign factor."""
+    import torch
+    from socu_amd import _lib
+
+    lib = _lib.load()
+    n, N, m, B = 40, 21, 3, 2
+    L0, E0, b0 = _stack(n, N, m, B, seed=12)
+    Lf, Ef = L0.copy(), E0.copy()
+    nd.factor(Lf, Ef)
+    xr = b0.copy()
+    nd.solve(Lf, Ef, xr)
+    L = torch.tensor(np.tril(Lf) + np.triu(np.ones_like(Lf), 1), device=device)
+    E = torch.tensor(Ef, device=device)
+    x = torch.tensor(b0, device=device)
+    work = torch.empty(lib.socu_solve_workspace_size(B, N, n, _lib.DTYPE_F64), dtype=torch.uint8, device=device)
+    st = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    _lib.check(lib.socu_cholesky_solve(L.data_ptr(), E.data_ptr(), x.data_ptr(), None, B, N, n, m, _lib.DTYPE_F64,
+                                       work.data_ptr(), st), "solve")
+    torch.cuda.synchronize()
+    assert _rel(x.cpu().numpy(), xr) < 1e-11

@@ -66,3 +66,23 @@ def test_invalid_arguments_fail_before_device_work():
     assert rc == _lib.EINVAL
     # B = 0 is a valid no-op
     assert lib.socu_cholesky_factor(None, None, None, 0, 5, 4, _lib.DTYPE_F64, None) == 0
+
+
+def test_substitutions_need_a_workspace():
+    lib = _lib.load()
+    assert lib.socu_solve_workspace_size(2, 512, 128, _lib.DTYPE_F64) == 2 * 512 * 128 * 128 * 8
+    assert lib.socu_solve_workspace_size(1, 7, 10, _lib.DTYPE_F32) == 7 * 10 * 10 * 4
+    assert lib.socu_solve_workspace_size(1, 0, 10, _lib.DTYPE_F32) == 0
+    p = ctypes.c_void_p(8)
+    for fn in (lib.socu_cholesky_solve, lib.socu_cholesky_forward_substitution,
+               lib.socu_cholesky_backward_substitution):
+        rc = fn(p, p, p, None, 1, 5, 4, 1, _lib.DTYPE_F64, None, None)
+        assert rc == _lib.EINVAL and "work" in _lib.last_error()
+        # B = 0: nothing to do, no workspace needed
+        assert fn(None, None, None, None, 0, 5, 4, 1, _lib.DTYPE_F64, None, None) == 0
+
+
+def test_spin_limit_setter_is_host_only():
+    lib = _lib.load()
+    assert lib.socu_set_sweep_spin_limit(-1) == 0
+    assert lib.socu_set_sweep_spin_limit(0) == 0

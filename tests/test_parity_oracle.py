@@ -132,3 +132,34 @@ def test_wide_level_factor_matches_oracle(n, dtype, device):
         for k in range(Er.shape[1]):
             if np.linalg.norm(Er[b, k]) > 0:
                 assert _rel(Eg[b, k], Er[b, k]) < tol, (b, k)
+
+
+@pytest.mark.parametrize("n,N,m", [(64, 20, 16), (64, 20, 17), (40, 13, 33), (128, 9, 33)])
+@pytest.mark.parametrize("op", ["factor_and_solve", "factor+solve"])
+def test_many_right_hand_sides_match_oracle(n, N, m, op, device):
+    """Right-hand sides beyond one 16-wide chunk (16, 17, 33): the sweep
+    kernels' chunk grid and a partial last chunk; E, tril(L) and x vs the
+    oracle, and the residual."""
+    import torch
+    from socu_amd.block_tridiag_solver import (
+        create_cholesky_factor_and_solve_launch,
+        create_cholesky_factor_launch,
+        create_cholesky_solve_launch,
+    )
+
+    L0, E0, b0, _, _ = nd.readme_problem(n, N, m, alpha=16.0, seed=21)
+    Lr, Er, xr = L0.copy(), E0.copy(), b0.copy()
+    nd.factor_and_solve(Lr, Er, xr)
+    L = torch.tensor(L0, device=device)
+    E = torch.tensor(E0, device=device)
+    x = torch.tensor(b0, device=device)
+    if op == "factor_and_solve":
+        create_cholesky_factor_and_solve_launch(L, E, x, use_cuda_graph=True)()
+    else:
+        create_cholesky_factor_launch(L, E)()
+        create_cholesky_solve_launch(L, E, x)()
+    assert _rel(np.tril(L.cpu().numpy()), np.tril(Lr)) < 1e-11
+    assert _rel(E.cpu().numpy(), Er) < 1e-11
+    xg = x.cpu().numpy()
+    assert _rel(xg, xr) < 1e-11
+    assert nd.rel_residual(L0, E0, xg, b0) <= 1e-12
