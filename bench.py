@@ -38,7 +38,12 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 # mixed-solve factor precisions (socu_amd.mixed) and their pivot / Schur types
-PRECISIONS = {"float32": "FP32 Schur, FP64 pivots", "float32_f32_pivots": "FP32 Schur, FP32 pivots"}
+PRECISIONS = {"float32": "FP32 Schur, FP64 pivots", "float32_f32_pivots": "FP32 Schur, FP32 pivots",
+              "bfloat16": "BF16 Schur (FP32 accumulation), FP64 pivots"}
+# refinement caps of the timed graphs (the graph replays every captured
+# iteration; converged ones are skipped kernel by kernel): FP32 factors need
+# two correction solves on the timed problem, BF16 Schur updates about eight
+DEFAULT_MAX_ITER = {"float32": 3, "float32_f32_pivots": 3, "bfloat16": 12}
 
 
 def log(msg):
@@ -55,7 +60,8 @@ def parse():
     ap.add_argument("--m", type=int, default=1)
     ap.add_argument("--alpha", type=float, default=16.0)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--max-iter", type=int, default=3)
+    ap.add_argument("--max-iter", type=int, default=0,
+                    help="refinement cap of the timed graphs (0: per precision, DEFAULT_MAX_ITER)")
     ap.add_argument("--precision", default="float32", choices=list(PRECISIONS),
                     help="factor precision of the timed mixed solve: FP32 Schur updates with FP64 diagonal-block "
                          "pivots (float32, the default), FP32 pivots, or BF16 Schur updates; the others are timed "
@@ -68,24 +74,63 @@ def parse():
     return ap.parse_args()
 
 
+def capi_factor_and_solve(lib, prefix, L, E, x, B, N, n, m, dtype, stream):
+    """One factor_and_solve through a library with the product's C-ABI shape
+    (include/socu_hip.h): libsocu_hip.so (prefix "socu_", device pointers and
+    a stream) or the oracle's C restatement (prefix "ndo_", host pointers)."""
+    rc = getattr(lib, prefix + "cholesky_factor_and_solve")(L, E, x, None, B, N, n, m, dtype, stream)
+    if rc != 0:
+        raise RuntimeError(f"{prefix}cholesky_factor_and_solve returned {rc}")
+
+
 def cpu_baseline(L0, E0, b0, seconds):
     """Time the oracle's C restatement (OpenMP over the nodes of a level) on
-    the same problem, repeated until `seconds` of CPU time have elapsed."""
+    the same problem through the C-ABI harness, repeated until `seconds` of
+    CPU time have elapsed."""
     from oracle import nd_c
+    from socu_amd import _lib
 
+    lib = nd_c.capi()
     threads = nd_c.set_threads(os.cpu_count() or 1)
+    N, n, m = b0.shape
     reps, t_total = 0, 0.0
     while t_total < seconds or reps < 2:
         L = L0[None].copy()
         E = E0[None].copy()
         x = b0[None].copy()
         t0 = time.perf_counter()
-        nd_c.factor_and_solve(L, E, x)
+        capi_factor_and_solve(lib, "ndo_", L.ctypes.data, E.ctypes.data, x.ctypes.data, 1, N, n, m, _lib.DTYPE_F64,
+                              None)
         t_total += time.perf_counter() - t0
         reps += 1
         if reps >= 200:
             break
     return reps / t_total, threads, reps
+
+
+def gpu_direct_ms(L0, E0, b0, dev, st, reps=10):
+    """The same harness on the GPU library's direct (eager, no plan or graph)
+    socu_cholesky_factor_and_solve, HIP events around each call."""
+    import torch
+
+    from socu_amd import _lib
+
+    lib = _lib.load()
+    N, n, m = b0.shape
+    Lp, Ep, bp = (torch.tensor(a, device=dev) for a in (L0, E0, b0))
+    L, E, x = Lp.clone(), Ep.clone(), bp.clone()
+
+    def step():
+        capi_factor_and_solve(lib, "socu_", L.data_ptr(), E.data_ptr(), x.data_ptr(), 1, N, n, m, _lib.DTYPE_F64,
+                              ctypes.c_void_p(st.cuda_stream))
+
+    def restore():
+        L.copy_(Lp)
+        E.copy_(Ep)
+        x.copy_(bp)
+
+    per, _ = _time_steps(step, restore, st, reps, 2)
+    return float(per.mean())
 
 
 def _profile(lib, fn, nclass):
@@ -347,7 +392,8 @@ def main():
     log(f"bench: fp64 factor_and_solve {r64}")
 
     # --- mixed factor + FP64 refinement: the metric -----------------------
-    rm, mx = bench_mixed(L0, E0, b0, Lp, Ep, dev, st, args.precision, args.tol, args.max_iter, args.steps,
+    cap = lambda prec: args.max_iter or DEFAULT_MAX_ITER[prec]  # noqa: E731
+    rm, mx = bench_mixed(L0, E0, b0, Lp, Ep, dev, st, args.precision, args.tol, cap(args.precision), args.steps,
                          args.warmup)
     msm = rm["ms"]
     log(f"bench: mixed+refine ({args.precision}) {rm}")
@@ -374,8 +420,12 @@ def main():
     for prec in PRECISIONS:
         if prec == args.precision:
             continue
-        rv, _ = bench_mixed(L0, E0, b0, Lp, Ep, dev, st, prec, args.tol, args.max_iter, args.steps, args.warmup)
+        rv, _ = bench_mixed(L0, E0, b0, Lp, Ep, dev, st, prec, args.tol, cap(prec), args.steps, args.warmup)
         rv["speedup_vs_fp64"] = round(ms64 / rv["ms"], 4)
+        need_v = max(rv["correction_solves"], 1)
+        if need_v < cap(prec):
+            rvn, _ = bench_mixed(L0, E0, b0, Lp, Ep, dev, st, prec, args.tol, need_v, args.steps, args.warmup)
+            rv["ms_with_max_iter_needed"] = rvn["ms"]
         variants[prec] = rv
         log(f"bench: mixed+refine ({prec}) {rv}")
 
@@ -386,7 +436,7 @@ def main():
         r8 = bench_fp64(L8, E8, b8, dev, st, args.secondary_steps, 3, profile=False)
         c3_m8 = {"workload": f"n={n} N={N} m=8 B=1, alpha={args.alpha}", "fp64": r8}
         for prec in PRECISIONS:
-            rv, _ = bench_mixed(L8, E8, b8, Lp, Ep, dev, st, prec, args.tol, args.max_iter, args.secondary_steps, 3)
+            rv, _ = bench_mixed(L8, E8, b8, Lp, Ep, dev, st, prec, args.tol, cap(prec), args.secondary_steps, 3)
             rv["speedup_vs_fp64"] = round(r8["ms"] / rv["ms"], 4)
             c3_m8[prec] = rv
         log(f"bench: C3 m=8 {c3_m8}")
@@ -400,7 +450,7 @@ def main():
         for alpha in (10.0, 2.0):
             L1, E1, b1 = problems.readme_problem(n, N, m, alpha=alpha, seed=0)
             L1t, E1t, x1 = torch.tensor(L1, device=dev), torch.tensor(E1, device=dev), torch.tensor(b1, device=dev)
-            m1 = create_cholesky_mixed_factor_and_solve_launch(L1t, E1t, x1, tol=args.tol, max_iter=args.max_iter)
+            m1 = create_cholesky_mixed_factor_and_solve_launch(L1t, E1t, x1, tol=args.tol, max_iter=cap("float32"))
             m1()
             rr1, it1 = m1.stats()
             xf = torch.tensor(b1, device=dev)
@@ -443,8 +493,10 @@ def main():
         log("bench: CPU baseline")
         rate, threads, creps = cpu_baseline(L0, E0, b0, args.cpu_seconds)
         cpu = {"value": round(rate, 4), "unit": "factor+solve/s", "cores": threads, "kind": "port",
-               "sample": f"oracle/nd_oracle.c FP64 factor_and_solve (reaches rel-resid < 1e-13), same "
-                         f"n={n} N={N} m={m} problem, {creps} runs"}
+               "sample": f"oracle/nd_oracle.c FP64 factor_and_solve (reaches rel-resid < 1e-13) through its "
+                         f"C-ABI-shaped entry point ndo_cholesky_factor_and_solve, same n={n} N={N} m={m} "
+                         f"problem, {creps} runs",
+               "gpu_same_harness_fp64_ms": round(gpu_direct_ms(L0, E0, b0, dev, st), 4)}
 
     secondary = None
     if not args.no_secondary:
@@ -465,7 +517,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"float32": "f32 Schur + f64 pivots, f64 refinement",
-                  "float32_f32_pivots": "f32 factor (f32 pivots), f64 refinement"}[args.precision],
+                  "float32_f32_pivots": "f32 factor (f32 pivots), f64 refinement",
+                  "bfloat16": "bf16 Schur (f32 acc) + f64 pivots, f64 refinement"}[args.precision],
         "data": "synthetic (README generator, seed 0)",
         "config": {"workload": f"ND block-tridiagonal Cholesky, mixed factor ({PRECISIONS[args.precision]}) + FP64 "
                                f"refinement to rel-resid {args.tol:g}, n={n} N={N} m={m} B=1, alpha={args.alpha} "

@@ -35,6 +35,12 @@ extern "C" {
 #define SOCU_DTYPE_F32 0
 #define SOCU_DTYPE_F64 1
 
+/* Factor precisions of FP32 storage (socu_cholesky_factor_f32, the
+ * mixed-precision solver); see there. */
+#define SOCU_PREC_FP32 0
+#define SOCU_PREC_FP32_F32PIV 1
+#define SOCU_PREC_BF16 2
+
 #define SOCU_OK 0
 #define SOCU_EINVAL -1      /* bad argument (shape, dtype, pointer) */
 #define SOCU_EUNSUPPORTED -2 /* valid but not supported (e.g. n > max block size) */
@@ -60,8 +66,22 @@ int socu_max_block_size(void);
 /* In-place factor of every unmasked system: tril(L[b,i]) <- diagonal Cholesky
  * blocks, E[b] <- ND off-diagonal factor blocks including fill-in.
  * Replaces the device work of create_cholesky_factor_launch's callback
- * (block_tridiag_solver.py:864-918).  E[b, N-1:] must be zero on entry. */
+ * (block_tridiag_solver.py:864-918).  The reference needs the fill-in region
+ * E[b, N-1:] zeroed before each factor (it accumulates F -= R Lc into it);
+ * here each fill-in block is written once as F = -R Lc, so its contents on
+ * entry are ignored (zeros give the same result). */
 int socu_cholesky_factor(void* L, void* E, const unsigned char* mask, int B, int N, int n, int dtype, void* stream);
+
+/* socu_cholesky_factor in FP32 storage with the precision options of the
+ * mixed-precision solver (SOCU_PREC_*, below): SOCU_PREC_FP32 factors and
+ * inverts the diagonal blocks in FP64 (rounded to FP32 on store), SOCU_PREC_BF16
+ * does so too and runs the Schur-complement products (syrk and fill-in gemm)
+ * on BF16 MFMA with FP32 accumulation, SOCU_PREC_FP32_F32PIV is the plain
+ * FP32 factor.  The reference's factor offers FP32 or FP64 end to end
+ * (block_tridiag_solver.py:26-39); this is the factor the mixed solver
+ * refines from, for callers that keep their own refinement. */
+int socu_cholesky_factor_f32(void* L, void* E, const unsigned char* mask, int B, int N, int n, int precision,
+                             void* stream);
 
 /* Substitutions read only tril(L) and E (as the reference's sweeps do,
  * socu/blas/potrf.py:101-103), so the factor may come from anywhere.  They
@@ -134,15 +154,15 @@ int socu_set_sweep_spin_limit(int polls);
  * FP64 solution on exit.  The handle owns an FP32 working copy of (L, E) that
  * is factored with FP32 MFMA Schur updates (precision SOCU_PREC_FP32: the
  * diagonal blocks are factored and inverted in FP64, then rounded;
- * SOCU_PREC_FP32_F32PIV: all in FP32), then runs up to max_iter correction
+ * SOCU_PREC_FP32_F32PIV: all in FP32; SOCU_PREC_BF16: FP64 diagonal blocks as
+ * SOCU_PREC_FP32, the Schur-complement products (syrk and fill-in gemm) on
+ * BF16 MFMA with FP32 accumulation), then runs up to max_iter correction
  * solves d = Lp32^-T Lp32^-1 r, x += d, r = b - Psi x (FP64), stopping per
  * system once ||r||_F <= tol ||b||_F.  The test runs on the device, so the
  * whole solve is one hipGraph (use_graph) with no host synchronisation.
  * socu_mixed_create allocates the workspace; execute allocates nothing.
  * socu_mixed_stats synchronises `stream` and returns, per system, the last
  * relative residual and the number of correction solves applied. */
-#define SOCU_PREC_FP32 0
-#define SOCU_PREC_FP32_F32PIV 1
 typedef struct socu_mixed socu_mixed;
 int socu_mixed_create(socu_mixed** out, const void* L, const void* E, void* x, const unsigned char* mask, int B, int N,
                       int n, int m, int precision, double tol, int max_iter, int use_graph);
@@ -172,7 +192,8 @@ int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask
 /* Batched uneven systems (no counterpart in the reference, whose launches
  * batch only systems sharing N and n: block_tridiag_solver.py:749-768).
  * Each entry is one independent system in the reference's layout: L (N,n,n),
- * E (calculate_off_diag_storage_len(N),n,n) with a zero fill-in region, and x
+ * E (calculate_off_diag_storage_len(N),n,n) (the fill-in region's contents on
+ * entry are ignored, as in socu_cholesky_factor), and x
  * (N,n,m) (ignored for SOCU_OP_FACTOR); `reserved` must be 0.  Needs
  * n <= socu_batched_max_block_size() and m <= 32.  mask is NULL or `count`
  * bytes indexed like `systems`.  One workgroup per system walks all its
@@ -218,6 +239,22 @@ int socu_batched_destroy(socu_batched* handle);
  * Masked-out systems keep their dL, dE. */
 int socu_adjoint_scatter(const void* bbar, const void* x, void* dL, void* dE, const unsigned char* mask, int B, int N,
                          int n, int m, int dtype, void* stream);
+
+/* Gradient with respect to the entries of a stored ND factor (no counterpart
+ * in the reference).  With Lp the ND-permuted Cholesky factor held in
+ * (tril(L), E) as the factor leaves it, writes G = -(u v^T + p q^T)
+ * restricted to the entries of Lp, in the factor's own storage:
+ *   dL_i = tril(G_ii) (diagonal included, strict upper triangle zero);
+ *   E entry k of level segment l (coupling a = (k+1)s-1, b = (k+2)s-1,
+ *   s = 2^l, block_tridiag_solver.py:92-145) holds Lp[b][a] for even k and
+ *   Lp[a][b]^T for odd k, so dE = -(u_b v_a^T + p_b q_a^T) for even k and
+ *   -(v_b u_a^T + q_b p_a^T) for odd k.
+ * u, v, p, q are (B,N,n,m) in the solver's vector layout; p and q may both be
+ * NULL.  For y = forward_substitution(b): u = backward_substitution(ybar),
+ * v = y.  For x = backward_substitution(y): u = x, v = forward_substitution(xbar).
+ * A solve is the sum of both.  Masked-out systems keep their dL, dE. */
+int socu_factor_adjoint_scatter(const void* u, const void* v, const void* p, const void* q, void* dL, void* dE,
+                                const unsigned char* mask, int B, int N, int n, int m, int dtype, void* stream);
 
 /* Last error message of the calling thread ("" if none). */
 const char* socu_last_error(void);

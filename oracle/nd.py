@@ -291,7 +291,7 @@ def _mask_idx(B, mask):
 # factor / forward / backward  (the ND algorithm)
 # ---------------------------------------------------------------------------
 
-def factor(L, E, mask=None):
+def factor(L, E, mask=None, schur_operand=None):
     """In-place ND block Cholesky (create_cholesky_factor_launch, :739-923).
 
     Per level l (stride s), for every node i, with all lower-level updates to
@@ -305,7 +305,11 @@ def factor(L, E, mask=None):
     of step 3's update (:88-102) is the same arithmetic reordered.
     Only tril(L) is meaningful afterwards.  Masked-out batch entries are not
     touched (README.md:214-235).
+    schur_operand (optional): a function applied to R and Lc where they enter
+    the Schur products of steps 2-4 (not where they are stored), to emulate a
+    factor whose Schur updates take rounded operands (e.g. BF16).
     """
+    op = schur_operand if schur_operand is not None else (lambda a: a)
     squeeze, (L4, E4) = _as4(L, E)
     B, N = L4.shape[:2]
     idx = _mask_idx(B, mask)
@@ -331,17 +335,26 @@ def factor(L, E, mask=None):
             has_r = i + s < N
             has_c = i >= s
             if has_r:
-                R = Eb[:, off + i // s]
+                R = op(Eb[:, off + i // s])
                 Lb[:, i + s] -= R @ _T(R)
             if has_c:
-                C = Eb[:, off + (i - s) // s]
+                C = op(Eb[:, off + (i - s) // s])
                 Lb[:, i - s] -= _T(C) @ C
             if has_r and has_c:
                 f = nxt + (i - s) // (2 * s)
-                Eb[:, f] -= Eb[:, off + i // s] @ Eb[:, off + (i - s) // s]
+                Eb[:, f] -= op(Eb[:, off + i // s]) @ op(Eb[:, off + (i - s) // s])
     L4[idx] = Lb
     E4[idx] = Eb
     return L
+
+
+def round_bf16(a):
+    """Rounds to the nearest BF16 value (ties to even), through FP32, as
+    float64 (the rounding a BF16 MFMA operand gets from FP32 storage)."""
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return (u.astype(np.uint32)).view(np.float32).astype(np.float64)
 
 
 def forward(L, E, x, mask=None):

@@ -82,3 +82,25 @@ def factor_and_solve(L, E, x, mask=None):
     mk = _mask(mask)
     B, N, n, m = x.shape
     load().ndo_factor_and_solve(_p(L), _p(E), _p(x), _p(mk), B, N, n, m)
+
+
+# The product's C-ABI shape (include/socu_hip.h), restated on host arrays by
+# the ndo_cholesky_* entry points: same argument lists as the socu_cholesky_*
+# direct entry points, so one harness drives either library.
+CAPI_ENTRY_POINTS = ("cholesky_factor", "cholesky_forward_substitution", "cholesky_backward_substitution",
+                     "cholesky_solve", "cholesky_factor_and_solve", "cholesky_solve_workspace_size")
+
+
+def capi():
+    """The library with the ndo_cholesky_* argument types set from the
+    product's own ctypes signatures (socu_amd._lib.SIGNATURES)."""
+    from socu_amd import _lib as plib
+
+    lib = load()
+    for name in CAPI_ENTRY_POINTS:
+        src = "socu_solve_workspace_size" if name == "cholesky_solve_workspace_size" else "socu_" + name
+        res, args = plib.SIGNATURES[src]
+        fn = getattr(lib, "ndo_" + name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib

@@ -269,3 +269,67 @@ int ndo_set_threads(int t) {
   return 1;
 #endif
 }
+
+/*
+ * The same computations behind the argument lists of libsocu_hip.so's direct
+ * entry points (include/socu_hip.h: socu_cholesky_factor, _solve,
+ * _forward_substitution, _backward_substitution, _factor_and_solve), on host
+ * arrays, so that one harness can drive either library with the same call:
+ * dtype must be 1 (float64; 0 = float32 is not restated and returns -2),
+ * work and stream are accepted and ignored (the arrays are host memory; the
+ * call returns when done), validation returns -1 like the product's
+ * SOCU_EINVAL.  Test infrastructure and CPU baseline only.
+ */
+static int capi_check(const void* L, const void* E, const void* x, int need_x, int B, int N, int n, int m, int dtype) {
+  if (dtype != 1) return dtype == 0 ? -2 : -1;
+  if (B < 0 || N < 1 || n < 1 || (need_x && m < 1)) return -1;
+  if (B > 0 && (L == NULL || (E == NULL && N > 1) || (need_x && x == NULL))) return -1;
+  return 0;
+}
+
+size_t ndo_cholesky_solve_workspace_size(int B, int N, int n, int dtype) {
+  (void)B, (void)N, (void)n, (void)dtype;
+  return 0;
+}
+
+int ndo_cholesky_factor(void* L, void* E, const unsigned char* mask, int B, int N, int n, int dtype, void* stream) {
+  (void)stream;
+  const int rc = capi_check(L, E, NULL, 0, B, N, n, 1, dtype);
+  if (rc == 0 && B > 0) ndo_factor((double*)L, (double*)E, mask, B, N, n);
+  return rc;
+}
+
+int ndo_cholesky_forward_substitution(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N,
+                                      int n, int m, int dtype, void* work, void* stream) {
+  (void)work, (void)stream;
+  const int rc = capi_check(L, E, x, 1, B, N, n, m, dtype);
+  if (rc == 0 && B > 0) ndo_forward((const double*)L, (const double*)E, (double*)x, mask, B, N, n, m);
+  return rc;
+}
+
+int ndo_cholesky_backward_substitution(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N,
+                                       int n, int m, int dtype, void* work, void* stream) {
+  (void)work, (void)stream;
+  const int rc = capi_check(L, E, x, 1, B, N, n, m, dtype);
+  if (rc == 0 && B > 0) ndo_backward((const double*)L, (const double*)E, (double*)x, mask, B, N, n, m);
+  return rc;
+}
+
+int ndo_cholesky_solve(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
+                       int dtype, void* work, void* stream) {
+  (void)work, (void)stream;
+  const int rc = capi_check(L, E, x, 1, B, N, n, m, dtype);
+  if (rc == 0 && B > 0) {
+    ndo_forward((const double*)L, (const double*)E, (double*)x, mask, B, N, n, m);
+    ndo_backward((const double*)L, (const double*)E, (double*)x, mask, B, N, n, m);
+  }
+  return rc;
+}
+
+int ndo_cholesky_factor_and_solve(void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
+                                  int dtype, void* stream) {
+  (void)stream;
+  const int rc = capi_check(L, E, x, 1, B, N, n, m, dtype);
+  if (rc == 0 && B > 0) ndo_factor_and_solve((double*)L, (double*)E, (double*)x, mask, B, N, n, m);
+  return rc;
+}

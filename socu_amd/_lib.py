@@ -34,6 +34,7 @@ KERNEL_CLASSES = ("potrf", "trsm", "schur", "forward", "backward", "refine")
 
 PREC_FP32 = 0
 PREC_FP32_F32PIV = 1
+PREC_BF16 = 2
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -53,6 +54,7 @@ SIGNATURES = {
     "socu_set_sweep_spin_limit": (_i, [_i]),
     "socu_solve_workspace_size": (ctypes.c_size_t, [_i, _i, _i, _i]),
     "socu_cholesky_factor": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _vp]),
+    "socu_cholesky_factor_f32": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _vp]),
     "socu_cholesky_forward_substitution": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp]),
     "socu_cholesky_backward_substitution": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp]),
     "socu_cholesky_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp]),
@@ -75,6 +77,7 @@ SIGNATURES = {
     "socu_batched_config": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "socu_batched_destroy": (_i, [_vp]),
     "socu_adjoint_scatter": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "socu_factor_adjoint_scatter": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "socu_last_error": (ctypes.c_char_p, []),
 }
 

@@ -6,8 +6,9 @@ reference's solver factors (``block_tridiag_solver.py:739-1727``): diagonal
 blocks built from the lower triangle of each ``L[i]`` (the upper triangle is
 ignored), coupling block ``E[i]`` at block position (i+1, i) and its
 transpose at (i, i+1).  The fill-in region of ``E`` (entries from N-1 on) is
-workspace: the forward pass factors a copy with that region cleared, so the
-inputs are never modified and the fill-in entries carry no gradient.
+workspace: the forward pass factors a copy of L and of the coupling blocks
+(the factor overwrites the fill-in region), so the inputs are never modified
+and the fill-in entries carry no gradient.
 
 Backward reuses the forward factor for the transpose solve
 bbar = Psi^{-1} xbar (Psi is symmetric) and then one HIP kernel scatters
@@ -54,7 +55,9 @@ class BlockTridiagSolve(torch.autograd.Function):
         code = _code(L)
         lib = _lib.load()
         Lw = L4.contiguous().clone()
-        Ew = torch.zeros_like(E4, memory_format=torch.contiguous_format)
+        # the fill-in region E[N-1:] is written by the factor before it is
+        # read, so only the coupling blocks are copied
+        Ew = torch.empty_like(E4, memory_format=torch.contiguous_format)
         if N > 1:
             Ew[:, : N - 1] = E4[:, : N - 1]
         x = b4.contiguous().clone()

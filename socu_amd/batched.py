@@ -5,7 +5,7 @@ The reference batches only systems that share the horizon N and block size n
 ``(B, N, n, n)``); MPC workloads hold thousands of systems of different N and
 n.  ``create_batched_cholesky_launch`` takes a list of systems, each in the
 reference's own layout (``L (N,n,n)``, ``E (calculate_off_diag_storage_len(N),
-n, n)`` with a zero fill-in region, ``x (N,n,m)``), and runs the chosen
+n, n)``, whose fill-in region the factor overwrites, ``x (N,n,m)``), and runs the chosen
 operation on all of them in one graph-capturable launch set.  The per-system
 result is the reference's factor / solve of that system
 (``block_tridiag_solver.py:739-1727``), in place, in the same storage.

@@ -249,8 +249,9 @@ def _make(op, name, L, E, x, mask, dtype, device, stream, use_cuda_graph, block_
 def create_cholesky_factor_launch(L, E, mask=None, block_dim=None, block_size=None, dtype=torch.float64, device=None,
                                   stream=None, use_cuda_graph=False):
     """In-place ND Cholesky factor of the block-tridiagonal matrix (L, E)
-    (block_tridiag_solver.py:739-923).  E[..., N-1:, :, :] must be zero before
-    each call (it receives the fill-in)."""
+    (block_tridiag_solver.py:739-923).  The fill-in region E[..., N-1:, :, :]
+    receives the fill-in; the reference needs it zeroed before each call, here
+    each fill-in block is written once and its contents on entry are ignored."""
     return _make(_lib.OP_FACTOR, "cholesky_factor", L, E, None, mask, dtype, device, stream, use_cuda_graph,
                  block_dim, block_size)
 

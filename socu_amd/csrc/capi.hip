@@ -321,6 +321,28 @@ int socu_cholesky_factor(void* L, void* E, const unsigned char* mask, int B, int
   return dispatch(SOCU_OP_FACTOR, L, E, nullptr, mask, B, N, n, 1, dtype, stream, nullptr);
 }
 
+int socu_cholesky_factor_f32(void* L, void* E, const unsigned char* mask, int B, int N, int n, int precision,
+                             void* stream) {
+  int rc = check_dims(L, E, nullptr, false, B, N, n, 1, SOCU_DTYPE_F32);
+  if (rc) return rc;
+  const int prec = precision == SOCU_PREC_FP32 ? socu::kPrecF64Pivots
+                   : precision == SOCU_PREC_BF16 ? socu::kPrecF64Pivots | socu::kPrecBf16Schur
+                   : precision == SOCU_PREC_FP32_F32PIV ? 0
+                                                        : -1;
+  if (prec < 0) return fail(SOCU_EINVAL, "unknown factor precision %d", precision);
+  if (B == 0) return SOCU_OK;
+  rc = ensure_init();
+  if (rc) return rc;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  const int Lv = socu::recursive_iterations(N);
+  for (int l = 0; l < Lv; ++l) {
+    const hipError_t e =
+        socu::factor_level_blocked<float>(static_cast<float*>(L), static_cast<float*>(E), mask, B, N, n, l, prec, st);
+    if (e != hipSuccess) return hip_fail(e, "factor level launch");
+  }
+  return SOCU_OK;
+}
+
 int socu_cholesky_forward_substitution(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N,
                                        int n, int m, int dtype, void* work, void* stream) {
   return dispatch(SOCU_OP_FORWARD, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream,
@@ -353,6 +375,20 @@ int socu_adjoint_scatter(const void* bbar, const void* x, void* dL, void* dE, co
   if (B == 0) return SOCU_OK;
   const hipError_t e = socu::adjoint_scatter(bbar, x, dL, dE, mask, B, N, n, m, dtype, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "adjoint scatter launch");
+  return SOCU_OK;
+}
+
+int socu_factor_adjoint_scatter(const void* u, const void* v, const void* p, const void* q, void* dL, void* dE,
+                                const unsigned char* mask, int B, int N, int n, int m, int dtype, void* stream) {
+  if (u == nullptr || v == nullptr || dL == nullptr || (dE == nullptr && N > 1))
+    return fail(SOCU_EINVAL, "u, v, dL or dE is NULL");
+  if ((p == nullptr) != (q == nullptr)) return fail(SOCU_EINVAL, "p and q must both be given or both be NULL");
+  int rc = check_dims(dL, dE, u, true, B, N, n, m, dtype);
+  if (rc) return rc;
+  if (B == 0) return SOCU_OK;
+  const hipError_t e =
+      socu::factor_adjoint_scatter(u, v, p, q, dL, dE, mask, B, N, n, m, dtype, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "factor adjoint scatter launch");
   return SOCU_OK;
 }
 
@@ -496,7 +532,7 @@ int socu_mixed_create(socu_mixed** out, const void* L, const void* E, void* x, c
   *out = nullptr;
   int rc = check_dims(L, E, x, true, B, N, n, m, SOCU_DTYPE_F64);
   if (rc) return rc;
-  if (precision != SOCU_PREC_FP32 && precision != SOCU_PREC_FP32_F32PIV)
+  if (precision != SOCU_PREC_FP32 && precision != SOCU_PREC_FP32_F32PIV && precision != SOCU_PREC_BF16)
     return fail(SOCU_EINVAL, "unknown factor precision %d", precision);
   if (!(tol > 0.0)) return fail(SOCU_EINVAL, "tolerance must be positive");
   if (max_iter < 1 || max_iter > 100) return fail(SOCU_EINVAL, "max_iter must be in [1, 100] (got %d)", max_iter);
@@ -517,7 +553,9 @@ int socu_mixed_create(socu_mixed** out, const void* L, const void* E, void* x, c
   h->p.m = m;
   h->p.tol = tol;
   h->p.max_iter = max_iter;
-  h->p.f64_pivots = precision == SOCU_PREC_FP32;
+  h->p.prec = precision == SOCU_PREC_FP32 ? socu::kPrecF64Pivots
+               : precision == SOCU_PREC_BF16 ? socu::kPrecF64Pivots | socu::kPrecBf16Schur
+                                             : 0;
   h->p.use_overlap = use_graph != 0;  // a second stream pays inside a graph only
   h->graph = nullptr;
   h->exec = nullptr;

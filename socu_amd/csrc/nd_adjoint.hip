@@ -1,18 +1,36 @@
-// Adjoint of x = Psi^{-1} b for the block-tridiagonal Psi of the ND solver
-// (diagonal blocks sym(L_i) built from the lower triangle of L_i, coupling
-// blocks E_i at (i+1, i) and E_i^T at (i, i+1)).  With bbar = Psi^{-1} xbar
-// (the transpose solve, made with the same factor since Psi is symmetric)
-// the gradient of the full matrix is G = -bbar x^T; restricted to the
-// parameters it is
+// Adjoint scatters of the ND solver (no counterpart in the reference, whose
+// launches are not differentiable).
+//
+// Mode kPsi -- x = Psi^{-1} b for the block-tridiagonal Psi (diagonal blocks
+// sym(L_i) built from the lower triangle of L_i, coupling blocks E_i at
+// (i+1, i) and E_i^T at (i, i+1)).  With bbar = Psi^{-1} xbar (the transpose
+// solve, made with the same factor since Psi is symmetric) the gradient of the
+// full matrix is G = -bbar x^T; restricted to the parameters it is
 //   dL_i[r][c] = -(bbar_i[r] . x_i[c] + x_i[r] . bbar_i[c])   r > c
 //   dL_i[r][r] = -(bbar_i[r] . x_i[r]),   dL_i[r][c] = 0       r < c
 //   dE_e[r][c] = -(bbar_{e+1}[r] . x_e[c] + x_{e+1}[r] . bbar_e[c])   e < N-1
 //   dE_e = 0 on the fill-in region (e >= N-1)
-// where "." sums over the m right-hand sides.  One workgroup computes one
-// 32 x 32 tile of one output block, staging the four vector slices it needs
-// through LDS in chunks of CH right-hand sides (8 for m <= 8: the staging
-// arrays then take 9 KiB instead of 34 KiB, and twice as many workgroups fit
-// on a CU); every output element is written exactly once.
+// where "." sums over the m right-hand sides.
+//
+// Mode kFactor -- the gradient with respect to the entries of the ND factor
+// Lp (PΨP^T = Lp Lp^T) as stored by the factor: G = -(u v^T + p q^T) on the
+// sparsity pattern of Lp, with u, v, p, q given in the solver's unpermuted
+// vector layout (p, q optional).  Diagonal block i: dL_i = tril(G_ii) (the
+// diagonal not halved: it holds factor entries, not a symmetric pair).  E
+// entry k of level segment l couples a = (k+1)s-1 and b = (k+2)s-1 (s = 2^l)
+// and holds Lp[b][a] for even k, Lp[a][b]^T for odd k
+// (block_tridiag_solver.py:92-145 index maps), so
+//   even k: dE = -(u_b v_a^T + p_b q_a^T),   odd k: dE = -(v_b u_a^T + q_b p_a^T).
+// Every E index is an entry of Lp (each is written by the factor), and the
+// strict upper triangle of each L_i gets zeros.  For y = Lp^{-1} P b the
+// gradient is G = -bbar y^T with bbar = Lp^{-T} ybar; for x = P^T Lp^{-T} y it
+// is G = -x w^T with w = Lp^{-1} xbar; a solve adds both.
+//
+// One workgroup computes one 32 x 32 tile of one output block, staging the
+// four vector slices it needs through LDS in chunks of CH right-hand sides (8
+// for m <= 8: the staging arrays then take 9 KiB instead of 34 KiB, and twice
+// as many workgroups fit on a CU); every output element is written exactly
+// once.
 #include "../../include/socu_hip.h"
 #include "nd_common.h"
 #include "nd_internal.h"
@@ -22,9 +40,21 @@ namespace socu {
 namespace {
 
 constexpr int kAdjTile = 32;
+enum { kPsi = 0, kFactor = 1 };
 
-template <typename T, int CH>
-__global__ __launch_bounds__(256) void k_adjoint_scatter(const T* __restrict__ bbar, const T* __restrict__ x,
+// Vector block of the (rows, cols) sides of the two products.  kPsi:
+// (u, v, p, q) = (bbar, x, x, bbar).
+template <typename T>
+struct AdjSrc {
+  const T* P;  // rows side, first product
+  const T* Q;  // cols side, first product
+  const T* R;  // rows side, second product (NULL: none)
+  const T* S;  // cols side, second product
+};
+
+template <typename T, int CH, int MODE>
+__global__ __launch_bounds__(256) void k_adjoint_scatter(const T* __restrict__ u, const T* __restrict__ v,
+                                                         const T* __restrict__ p, const T* __restrict__ q,
                                                          T* __restrict__ dL, T* __restrict__ dE,
                                                          const uint8_t* __restrict__ mask, int N, int n, int m,
                                                          long long nE, int tiles1d) {
@@ -44,34 +74,56 @@ __global__ __launch_bounds__(256) void k_adjoint_scatter(const T* __restrict__ b
   const int tid = threadIdx.x;
   const int tr = tid / 8, tc8 = tid % 8;  // thread owns row r0+tr, cols c0+tc8+8q
   if (masked_out(mask, b)) return;  // masked systems keep their outputs
-  const bool fill = !isL && e >= N - 1;
+  const bool fill = MODE == kPsi && !isL && e >= N - 1;
   T acc[4] = {T(0), T(0), T(0), T(0)};
   if (!fill) {
     const size_t nm = (size_t)n * m;
-    const T* bb = bbar + (size_t)b * N * nm;
-    const T* xx = x + (size_t)b * N * nm;
-    // (rows, cols) sources: dL_i: (bbar_i, x_i) + (x_i, bbar_i);
-    // dE_e: (bbar_{e+1}, x_e) + (x_{e+1}, bbar_e)
-    const size_t ra = isL ? (size_t)blk : (size_t)(e + 1), ca = isL ? (size_t)blk : (size_t)e;
-    const T* pr = bb + ra * nm;
-    const T* qc = xx + ca * nm;
-    const T* rr = xx + ra * nm;
-    const T* sc = bb + ca * nm;
+    const size_t sys = (size_t)b * N * nm;
+    // row / column vector blocks and which inputs feed which side
+    size_t ra, ca;
+    bool swap = false;  // odd factor entries: rows from (v, q), cols from (u, p)
+    if (isL) {
+      ra = ca = (size_t)blk;
+    } else if (MODE == kPsi) {
+      ra = (size_t)(e + 1);
+      ca = (size_t)e;
+    } else {
+      long long off = 0;
+      int l = 0;
+      while (off + (N >> l) - 1 <= e) {
+        off += (N >> l) - 1;
+        ++l;
+      }
+      const long long k = e - off;
+      const long long s = 1ll << l;
+      ca = (size_t)((k + 1) * s - 1);  // a
+      ra = (size_t)((k + 2) * s - 1);  // b
+      swap = (k & 1) != 0;
+    }
+    AdjSrc<T> src;
+    if (MODE == kPsi) {
+      src = {u + sys + ra * nm, v + sys + ca * nm, p + sys + ra * nm, q + sys + ca * nm};
+    } else if (!swap) {
+      src = {u + sys + ra * nm, v + sys + ca * nm, p ? p + sys + ra * nm : nullptr, p ? q + sys + ca * nm : nullptr};
+    } else {
+      src = {v + sys + ra * nm, u + sys + ca * nm, p ? q + sys + ra * nm : nullptr, p ? p + sys + ca * nm : nullptr};
+    }
+    const bool two = src.R != nullptr;
     for (int k0 = 0; k0 < m; k0 += CH) {
       const int kc = min(CH, m - k0);
       for (int idx = tid; idx < kAdjTile * CH; idx += 256) {
         const int i = idx / CH, k = idx % CH;
         const bool okr = r0 + i < n && k < kc, okc = c0 + i < n && k < kc;
-        P[i][k] = okr ? pr[(size_t)(r0 + i) * m + k0 + k] : T(0);
-        R[i][k] = okr ? rr[(size_t)(r0 + i) * m + k0 + k] : T(0);
-        Q[i][k] = okc ? qc[(size_t)(c0 + i) * m + k0 + k] : T(0);
-        S[i][k] = okc ? sc[(size_t)(c0 + i) * m + k0 + k] : T(0);
+        P[i][k] = okr ? src.P[(size_t)(r0 + i) * m + k0 + k] : T(0);
+        Q[i][k] = okc ? src.Q[(size_t)(c0 + i) * m + k0 + k] : T(0);
+        R[i][k] = okr && two ? src.R[(size_t)(r0 + i) * m + k0 + k] : T(0);
+        S[i][k] = okc && two ? src.S[(size_t)(c0 + i) * m + k0 + k] : T(0);
       }
       __syncthreads();
       for (int k = 0; k < kc; ++k) {
-        const T p = P[tr][k], q = R[tr][k];
+        const T pv = P[tr][k], rv = R[tr][k];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] += p * Q[tc8 + 8 * j][k] + q * S[tc8 + 8 * j][k];
+        for (int j = 0; j < 4; ++j) acc[j] += pv * Q[tc8 + 8 * j][k] + rv * S[tc8 + 8 * j][k];
       }
       __syncthreads();
     }
@@ -82,32 +134,38 @@ __global__ __launch_bounds__(256) void k_adjoint_scatter(const T* __restrict__ b
     for (int j = 0; j < 4; ++j) {
       const int c = c0 + tc8 + 8 * j;
       if (c >= n) continue;
-      T v = fill ? T(0) : -acc[j];
-      if (isL) v = (r > c) ? v : ((r == c) ? T(0.5) * v : T(0));
-      out[(size_t)r * n + c] = v;
+      T val = fill ? T(0) : -acc[j];
+      if (isL) val = (r > c) ? val : ((r == c) ? (MODE == kPsi ? T(0.5) * val : val) : T(0));
+      out[(size_t)r * n + c] = val;
     }
   }
+}
+
+template <typename T, int MODE>
+void launch_adjoint(const void* u, const void* v, const void* p, const void* q, void* dL, void* dE,
+                    const uint8_t* mask, int B, int N, int n, int m, hipStream_t st) {
+  const long long nE = off_diag_storage_len(N);
+  const int tiles1d = (n + kAdjTile - 1) / kAdjTile;
+  const long long gx = (N + nE) * tiles1d * tiles1d;
+  auto k = m <= 8 ? k_adjoint_scatter<T, 8, MODE> : k_adjoint_scatter<T, 32, MODE>;
+  hipLaunchKernelGGL(k, dim3((unsigned)gx, B), dim3(256), 0, st, static_cast<const T*>(u), static_cast<const T*>(v),
+                     static_cast<const T*>(p), static_cast<const T*>(q), static_cast<T*>(dL), static_cast<T*>(dE),
+                     mask, N, n, m, nE, tiles1d);
 }
 
 }  // namespace
 
 hipError_t adjoint_scatter(const void* bbar, const void* x, void* dL, void* dE, const uint8_t* mask, int B, int N,
                            int n, int m, int dtype, hipStream_t st) {
-  const long long nE = off_diag_storage_len(N);
-  const int tiles1d = (n + kAdjTile - 1) / kAdjTile;
-  const long long gx = (N + nE) * tiles1d * tiles1d;
-  const bool narrow = m <= 8;
-  if (dtype == SOCU_DTYPE_F64) {
-    auto k = narrow ? k_adjoint_scatter<double, 8> : k_adjoint_scatter<double, 32>;
-    hipLaunchKernelGGL(k, dim3((unsigned)gx, B), dim3(256), 0, st, static_cast<const double*>(bbar),
-                       static_cast<const double*>(x), static_cast<double*>(dL), static_cast<double*>(dE), mask, N, n,
-                       m, nE, tiles1d);
-  } else {
-    auto k = narrow ? k_adjoint_scatter<float, 8> : k_adjoint_scatter<float, 32>;
-    hipLaunchKernelGGL(k, dim3((unsigned)gx, B), dim3(256), 0, st, static_cast<const float*>(bbar),
-                       static_cast<const float*>(x), static_cast<float*>(dL), static_cast<float*>(dE), mask, N, n, m,
-                       nE, tiles1d);
-  }
+  if (dtype == SOCU_DTYPE_F64) launch_adjoint<double, kPsi>(bbar, x, x, bbar, dL, dE, mask, B, N, n, m, st);
+  else launch_adjoint<float, kPsi>(bbar, x, x, bbar, dL, dE, mask, B, N, n, m, st);
+  return hipGetLastError();
+}
+
+hipError_t factor_adjoint_scatter(const void* u, const void* v, const void* p, const void* q, void* dL, void* dE,
+                                  const uint8_t* mask, int B, int N, int n, int m, int dtype, hipStream_t st) {
+  if (dtype == SOCU_DTYPE_F64) launch_adjoint<double, kFactor>(u, v, p, q, dL, dE, mask, B, N, n, m, st);
+  else launch_adjoint<float, kFactor>(u, v, p, q, dL, dE, mask, B, N, n, m, st);
   return hipGetLastError();
 }
 

@@ -119,30 +119,12 @@ __device__ __forceinline__ void tprod_pos(int e, int& i, int& j) {
   }
 }
 
-// Values of block src (n x n) at this thread's tprod output positions
-// (0 outside n); unconditional loads, pinned by the caller's group.
+// Zeros at this thread's tprod output positions: the start of a fill-in
+// product (the block receives its only update there).
 template <typename T, int NT, int W>
-__device__ __forceinline__ void pfetch(T* v, const T* __restrict__ src, int n) {
-  constexpr int EPL = BShape<NT, W>::EPL;
-  if (src == nullptr) return;
+__device__ __forceinline__ void pzero(T* v) {
 #pragma unroll
-  for (int e = 0; e < EPL; ++e) {
-    int i, j;
-    tprod_pos<T, NT, W>(e, i, j);
-    v[e] = src[(size_t)min(i, n - 1) * n + min(j, n - 1)];
-  }
-}
-
-template <typename T, int NT, int W>
-__device__ __forceinline__ void pcommit(T* v, const T* src, int n) {
-  constexpr int EPL = BShape<NT, W>::EPL;
-#pragma unroll
-  for (int e = 0; e < EPL; ++e) {
-    int i, j;
-    tprod_pos<T, NT, W>(e, i, j);
-    if (src != nullptr) pin_reg(v[e]);
-    v[e] = (src != nullptr && i < n && j < n) ? v[e] : T(0);
-  }
+  for (int e = 0; e < BShape<NT, W>::EPL; ++e) v[e] = T(0);
 }
 
 // Vector block (n x m, row-major) into a tile of pitch yp; rows >= n zero.
@@ -354,10 +336,10 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
             if (fac && (up || node)) tload<T, NT, W>(Dt, srcD, n, q, false);
           }
         }
-        // level > 0: the old couplings of node k are read at the fill
-        // products' output positions (the fill-in region is zero on entry; it
-        // is read for parity with the accumulating reference update), each
-        // just before the product that needs it
+        // level > 0: node k's couplings are fill-in blocks that receive their
+        // only update here (the reference's F -= R Lc on a zeroed block), so
+        // the fill products start from zero and the fill-in region's contents
+        // on entry are never read
         T old[EPL];
         if (fwd) {
           yload<T, NT, W>(Y0, yp, yk, n, m, q);
@@ -383,8 +365,7 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
                 right, n, [&](int i, int kk) { return Ct[kk * TP + i]; },
                 [&](int kk, int j) { return Y2[kk * yp + j]; }, [&](int i, int j, T v) { Y0[i * yp + j] -= v; });
           if (fac && has_lc) {
-            pfetch<T, NT, W>(old, Lck, n);
-            pcommit<T, NT, W>(old, Lck, n);
+            pzero<T, NT, W>(old);
             tprod1<T, NT, W>(
                 [&](int i, int kk) { return Bt[i * TP + kk]; }, [&](int kk, int j) { return Dt[kk * TP + j]; },
                 [&](int i, int j, T v) {
@@ -406,9 +387,8 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
         if (up && fac) {  // stage R_{k+h}; fetch the old R_k with it
           T vb[EPL];
           tfetch<T, NT, W>(vb, Rr, n, q);
-          if (has_r) pfetch<T, NT, W>(old, Rk, n);
           tcommit<T, NT, W>(Bt, vb, Rr, n, q, false);
-          if (has_r) pcommit<T, NT, W>(old, Rk, n);
+          if (has_r) pzero<T, NT, W>(old);
         }
         if (defer) {
           if (fac)

@@ -745,8 +745,9 @@ __device__ __forceinline__ void store_chunk64(const Chunk64<T>& ch, T* As, T* Bs
   }
 }
 
-// Sequence of up to two products over K chunks: acc += sum_p A_p B_p.
-template <typename T>
+// Sequence of up to two products over K chunks: acc += sum_p A_p B_p (BF:
+// on BF16 MFMA, float storage).
+template <typename T, bool BF>
 __device__ __forceinline__ void products64(typename Mfma<T>::acc_t acc[2][2], bool active, int np, const T* const* Am,
                                            const T* const* Bm, const int* form, int n, int r0, int c0, T* As, T* Bs) {
   const int wave = wave_id();
@@ -764,11 +765,11 @@ __device__ __forceinline__ void products64(typename Mfma<T>::acc_t acc[2][2], bo
       const int p1 = (q + 1) / nk, k1 = ((q + 1) - p1 * nk) * kSK;
       load_chunk64<T>(ch, Am[p1], Bm[p1], form[p1], n, r0, c0, k1);
     }
-    if (active) wave_gemm<T>(acc, As + 32 * wi, 1, kSP, Bs + 32 * wj, kSP, 1, kSK);
+    if (active) schur_gemm<T, BF>(acc, As + 32 * wi, 1, kSP, Bs + 32 * wj, kSP, 1, kSK);
   }
 }
 
-template <typename T, int NF>
+template <typename T, int NF, bool BF = false>
 __global__ __launch_bounds__(kThreads) void k_schur_blk(T* __restrict__ L, T* __restrict__ E,
                                                        const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
                                                        int off, int next_off, int nT, int ntile) {
@@ -800,7 +801,7 @@ __global__ __launch_bounds__(kThreads) void k_schur_blk(T* __restrict__ L, T* __
     const T* Lc = (j + s < N) ? Eb + (size_t)(off + j / s) * nn : R;
     const T* Am[2] = {R, Lc};
     const int form[2] = {kABt, kAtB};
-    products64<T>(acc, active, (j + s < N) ? 2 : 1, Am, Am, form, n, tr * kSTile, tc * kSTile, As, Bs);
+    products64<T, BF>(acc, active, (j + s < N) ? 2 : 1, Am, Am, form, n, tr * kSTile, tc * kSTile, As, Bs);
   } else {
     task -= nT * ntri;
     const int f = task / (ntile * ntile);
@@ -814,14 +815,23 @@ __global__ __launch_bounds__(kThreads) void k_schur_blk(T* __restrict__ L, T* __
     const T* Am[1] = {R};
     const T* Bm[1] = {Lc};
     const int form[1] = {kAB};
-    products64<T>(acc, active, 1, Am, Bm, form, n, tr * kSTile, tc * kSTile, As, Bs);
+    products64<T, BF>(acc, active, 1, Am, Bm, form, n, tr * kSTile, tc * kSTile, As, Bs);
   }
   if (!active) return;
   const int rb = tr * kSTile + 32 * wi, cb = tc * kSTile + 32 * wj;
-  acc_foreach<T>(acc, [&](int r, int c, T v) {
-    const int rr = rb + r, cc = cb + c;
-    if (rr < n && cc < n) C[(size_t)rr * n + cc] -= v;
-  });
+  if (blockIdx.x < nT * ntri) {
+    acc_foreach<T>(acc, [&](int r, int c, T v) {
+      const int rr = rb + r, cc = cb + c;
+      if (rr < n && cc < n) C[(size_t)rr * n + cc] -= v;
+    });
+  } else {
+    // the fill-in block receives its only update here: F = -R Lc, whatever
+    // it held on entry (the reference's F -= R Lc on a zeroed block)
+    acc_foreach<T>(acc, [&](int r, int c, T v) {
+      const int rr = rb + r, cc = cb + c;
+      if (rr < n && cc < n) C[(size_t)rr * n + cc] = -v;
+    });
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -877,7 +887,7 @@ __device__ __forceinline__ void panel_store(T* P, const T* v, bool cols, int i0,
   }
 }
 
-template <typename T, int NB, int NF>
+template <typename T, int NB, int NF, bool BF = false>
 __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* __restrict__ E,
                                                          const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
                                                          int off, int next_off, int nT) {
@@ -940,11 +950,15 @@ __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* 
 #pragma unroll
   for (int p = 0; p < 4; ++p)
     if (p < 2 * np) panel_load<T, NB>(v[p], M[p], cols[p], i0[p], n);
-  T cv[4];
+  // C values: the target's (the fill-in block receives its only update here,
+  // F = -R Lc, so its entry contents are not read)
+  T cv[4] = {T(0), T(0), T(0), T(0)};
+  if (blockIdx.x < nT * NTRI) {
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int rr = 32 * tr + 16 * wi + Mfma<T>::row(l, r), cc = 32 * tc + 16 * wj + Mfma<T>::col(l, r);
-    cv[r] = C[min(rr, n - 1) * n + min(cc, n - 1)];
+    for (int r = 0; r < 4; ++r) {
+      const int rr = 32 * tr + 16 * wi + Mfma<T>::row(l, r), cc = 32 * tc + 16 * wj + Mfma<T>::col(l, r);
+      cv[r] = C[min(rr, n - 1) * n + min(cc, n - 1)];
+    }
   }
 #pragma unroll
   for (int p = 0; p < 4; ++p)
@@ -958,8 +972,8 @@ __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* 
   __syncthreads();
   typename Mfma<T>::acc_t acc = acc_zero<T>();
   for (int p = 0; p < np; ++p)
-    wave_gemm16<T>(acc, Pn + (2 * p) * 32 * PP + 16 * wi * PP, PP, 1, Pn + (2 * p + 1) * 32 * PP + 16 * wj * PP, 1,
-                   PP, NP);
+    schur_gemm16<T, BF>(acc, Pn + (2 * p) * 32 * PP + 16 * wi * PP, PP, 1, Pn + (2 * p + 1) * 32 * PP + 16 * wj * PP,
+                        1, PP, NP);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int rr = 32 * tr + 16 * wi + Mfma<T>::row(l, r), cc = 32 * tc + 16 * wj + Mfma<T>::col(l, r);
@@ -1276,11 +1290,19 @@ static int trsm_chunks_per_block() {
 
 template <typename T, int NB>
 static hipError_t set_schur_small_attrs() {
-  hipError_t e = hipFuncSetAttribute((const void*)k_schur_small<T, NB, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)schur_small_lds<T, NB>());
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)k_schur_small<T, NB, 32 * NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)schur_small_lds<T, NB>());
+  const int lds = (int)schur_small_lds<T, NB>();
+  hipError_t e = hipFuncSetAttribute((const void*)k_schur_small<T, NB, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_schur_small<T, NB, 32 * NB>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if constexpr (sizeof(T) == 4) {  // BF16 products (float storage only)
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)k_schur_small<T, NB, 0, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lds);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)k_schur_small<T, NB, 32 * NB, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  }
+  return e;
 }
 
 // Whether a level's Schur updates (nodes = targets + fill-ins over the batch)
@@ -1356,7 +1378,7 @@ hipError_t init_blocked_kernels() {
 }
 
 template <typename T, int NB, int NF>
-static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, bool f64_pivots,
+static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, int prec,
                                   hipStream_t st, hipEvent_t after_potrf, hipEvent_t after_trsm,
                                   hipEvent_t before_trsm) {
   const int nE = (int)off_diag_storage_len(N);
@@ -1365,7 +1387,8 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
   if (lv.dim <= 0 || B <= 0) return hipSuccess;
   const size_t lds_trsm = trsm_blk_lds<T, NB>();
   kt_begin(kKcPotrf, st);
-  if (f64_pivots && sizeof(T) == 4) {
+  const bool bf16 = (prec & kPrecBf16Schur) != 0 && sizeof(T) == 4;
+  if ((prec & kPrecF64Pivots) != 0 && sizeof(T) == 4) {
     const size_t lds_potrf = potrf_blk_lds<double, NB>();
     hipLaunchKernelGGL((k_potrf_blk<T, NB, NF, double>), dim3(lv.dim, B), dim3(potrf_threads(NB)), lds_potrf, st, L, mask, N,
                        n, lv.s);
@@ -1412,11 +1435,17 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
     if (use_schur_small(B * (nT + nF), n, (int)sizeof(T))) {
       const int nsmall = nT * (NB * (NB + 1) / 2) + nF * NB * NB;
       const size_t lds_small = schur_small_lds<T, NB>();
-      hipLaunchKernelGGL((k_schur_small<T, NB, NF>), dim3(nsmall, B), dim3(kThreads), lds_small, st, L, E, mask, N, n,
-                         nE, lv.s, lv.off, lv.next_off, nT);
+      auto k = k_schur_small<T, NB, NF, false>;
+      if constexpr (sizeof(T) == 4)
+        if (bf16) k = k_schur_small<T, NB, NF, true>;
+      hipLaunchKernelGGL(k, dim3(nsmall, B), dim3(kThreads), lds_small, st, L, E, mask, N, n, nE, lv.s, lv.off,
+                         lv.next_off, nT);
     } else {
-      hipLaunchKernelGGL((k_schur_blk<T, NF>), dim3(tasks, B), dim3(kThreads), 0, st, L, E, mask, N, n, nE, lv.s,
-                         lv.off, lv.next_off, nT, ntile);
+      auto k = k_schur_blk<T, NF, false>;
+      if constexpr (sizeof(T) == 4)
+        if (bf16) k = k_schur_blk<T, NF, true>;
+      hipLaunchKernelGGL(k, dim3(tasks, B), dim3(kThreads), 0, st, L, E, mask, N, n, nE, lv.s, lv.off, lv.next_off, nT,
+                         ntile);
     }
     kt_end(kKcSchur, st);
   }
@@ -1424,10 +1453,10 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
 }
 
 template <typename T>
-hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, bool f64_pivots,
+hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, int prec,
                                 hipStream_t st, hipEvent_t after_potrf, hipEvent_t after_trsm,
                                 hipEvent_t before_trsm) {
-  const bool fp = f64_pivots;
+  const int fp = prec;
   switch (n) {
     case 32: return factor_level_nb<T, 1, 32>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
     case 64: return factor_level_nb<T, 2, 64>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
@@ -1601,7 +1630,7 @@ void overlap_destroy(SweepOverlap& ov) {
 // does not touch.
 template <typename T>
 hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, const uint8_t* sweep_mask, int B, int N,
-                               int n, int m, bool f64_pivots, const SweepOverlap& ov, hipStream_t st,
+                               int n, int m, int prec, const SweepOverlap& ov, hipStream_t st,
                                const std::function<hipError_t(hipStream_t)>* side_first) {
   const int Lv = recursive_iterations(N);
   hipError_t e = hipEventRecord(ov.fork, st);
@@ -1610,7 +1639,7 @@ hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, con
   if (e == hipSuccess && side_first != nullptr) e = hipEventRecord(ov.ready, ov.side);
   for (int l = 0; l < Lv && e == hipSuccess; ++l) {
     const bool last = l == Lv - 1;
-    e = factor_level_blocked<T>(L, E, factor_mask, B, N, n, l, f64_pivots, st, ov.potrf[l],
+    e = factor_level_blocked<T>(L, E, factor_mask, B, N, n, l, prec, st, ov.potrf[l],
                                 last ? nullptr : ov.trsm[l], (l == 0 && side_first != nullptr) ? ov.ready : nullptr);
     if (e == hipSuccess) e = hipStreamWaitEvent(ov.side, ov.potrf[l], 0);
     if (e == hipSuccess) e = forward_apply_blocked<T>(L, E, x, sweep_mask, B, N, n, m, l, ov.side);
@@ -1623,17 +1652,17 @@ hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, con
 }
 
 template hipError_t factor_with_forward<float>(float*, float*, float*, const uint8_t*, const uint8_t*, int, int, int,
-                                               int, bool, const SweepOverlap&, hipStream_t,
+                                               int, int, const SweepOverlap&, hipStream_t,
                                                const std::function<hipError_t(hipStream_t)>*);
 template hipError_t factor_with_forward<double>(double*, double*, double*, const uint8_t*, const uint8_t*, int, int,
-                                                int, int, bool, const SweepOverlap&, hipStream_t,
+                                                int, int, int, const SweepOverlap&, hipStream_t,
                                                 const std::function<hipError_t(hipStream_t)>*);
 
 template hipError_t trtri_blocked<float>(const float*, float*, const uint8_t*, int, int, int, hipStream_t);
 template hipError_t trtri_blocked<double>(const double*, double*, const uint8_t*, int, int, int, hipStream_t);
-template hipError_t factor_level_blocked<float>(float*, float*, const uint8_t*, int, int, int, int, bool,
+template hipError_t factor_level_blocked<float>(float*, float*, const uint8_t*, int, int, int, int, int,
                                                 hipStream_t, hipEvent_t, hipEvent_t, hipEvent_t);
-template hipError_t factor_level_blocked<double>(double*, double*, const uint8_t*, int, int, int, int, bool,
+template hipError_t factor_level_blocked<double>(double*, double*, const uint8_t*, int, int, int, int, int,
                                                  hipStream_t, hipEvent_t, hipEvent_t, hipEvent_t);
 template hipError_t forward_step_blocked<float>(const float*, const float*, float*, const uint8_t*, int, int, int, int,
                                                 int, hipStream_t, bool);

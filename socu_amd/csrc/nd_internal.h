@@ -30,15 +30,22 @@ void kt_end(int cls, hipStream_t st);
 // before any stream capture.
 hipError_t init_blocked_kernels();
 
+// Precision options of a factor in float storage (bit flags; ignored for
+// double storage):
+//   kPrecF64Pivots: the diagonal blocks are factored and inverted in FP64 and
+//     rounded to FP32 on store;
+//   kPrecBf16Schur: the Schur-complement products (syrk and the fill-in gemm)
+//     run on BF16 MFMA with FP32 accumulation, operands rounded from FP32.
+constexpr int kPrecF64Pivots = 1;
+constexpr int kPrecBf16Schur = 2;
+
 // Kernels of one ND factor level (potrf, then trsm and Schur updates unless it
-// is the last level), for block size n <= kMaxN.
-// f64_pivots (float storage only): the diagonal blocks are factored and
-// inverted in FP64 and rounded to FP32 on store.
+// is the last level), for block size n <= kMaxN; prec: kPrec* flags.
 // after_potrf / after_trsm (optional) are recorded on st after the level's
 // potrf and trsm launches (the forward sweep can follow them on another
 // stream); st waits for before_trsm (optional) between potrf and trsm.
 template <typename T>
-hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, bool f64_pivots,
+hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, int prec,
                                 hipStream_t st, hipEvent_t after_potrf = nullptr, hipEvent_t after_trsm = nullptr,
                                 hipEvent_t before_trsm = nullptr);
 
@@ -126,7 +133,7 @@ void overlap_destroy(SweepOverlap& ov);
 // waits for it (it may prepare E, and anything the sweep reads).
 template <typename T>
 hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, const uint8_t* sweep_mask, int B, int N,
-                               int n, int m, bool f64_pivots, const SweepOverlap& ov, hipStream_t st,
+                               int n, int m, int prec, const SweepOverlap& ov, hipStream_t st,
                                const std::function<hipError_t(hipStream_t)>* side_first = nullptr);
 
 // Mixed-precision factor_and_solve with FP64 refinement (nd_refine.hip).
@@ -138,7 +145,7 @@ struct MixedPlan {
   int B = 0, N = 0, n = 0, m = 0, nE = 0;
   double tol = 1e-10;
   int max_iter = 10;
-  bool f64_pivots = true;
+  int prec = kPrecF64Pivots;  // kPrec* flags of the FP32 factor
   // workspace
   float* L32 = nullptr;
   float* E32 = nullptr;
@@ -190,5 +197,9 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st);
 // bbar = Psi^{-1} xbar and x; all arrays in the solver's (B, ...) layout.
 hipError_t adjoint_scatter(const void* bbar, const void* x, void* dL, void* dE, const uint8_t* mask, int B, int N,
                            int n, int m, int dtype, hipStream_t st);
+// Gradient with respect to the stored ND factor entries:
+// G = -(u v^T + p q^T) on the pattern of Lp (p, q may both be NULL).
+hipError_t factor_adjoint_scatter(const void* u, const void* v, const void* p, const void* q, void* dL, void* dE,
+                                  const uint8_t* mask, int B, int N, int n, int m, int dtype, hipStream_t st);
 
 }  // namespace socu

@@ -23,7 +23,9 @@ namespace socu {
 
 // FP32 working copy of Psi in one launch: per batch entry
 //   tril(L32) <- tril(L) (lc elements), E32[0, ec) <- E (the tridiagonal
-//   blocks), E32[ec, es) <- 0 (fill-in region).
+//   blocks); es is the per-entry stride of E and E32.  The fill-in region
+//   E32[ec, es) is not touched: the factor writes each of its blocks (once,
+//   as F = -R Lc) before anything reads it.
 // The strict upper triangle of L32 is left as it is: the factor kernels read
 // only tril of a diagonal block and then store its inverse there (the
 // reference also reads only tril(L), potrf.py:101-103), so half of L is
@@ -39,7 +41,7 @@ __global__ __launch_bounds__(kThreads) void k_psi_to_f32(float* __restrict__ L32
   const int b = blockIdx.y;
   if (masked_out(mask, b)) return;
   constexpr int W = VEC ? 4 : 1;
-  const size_t nl = lc / W, ne = ec / W, total = (lc + es) / W;
+  const size_t nl = lc / W, total = (lc + ec) / W;
   const double* Lb = L + (size_t)b * lc;
   const double* Eb = E + (size_t)b * es;
   float* L32b = L32 + (size_t)b * lc;
@@ -52,21 +54,18 @@ __global__ __launch_bounds__(kThreads) void k_psi_to_f32(float* __restrict__ L32
         const double2 a = reinterpret_cast<const double2*>(Lb)[2 * u];
         const double2 c = reinterpret_cast<const double2*>(Lb)[2 * u + 1];
         reinterpret_cast<float4*>(L32b)[u] = make_float4((float)a.x, (float)a.y, (float)c.x, (float)c.y);
-      } else if (u - nl < ne) {
+      } else {
         const size_t v = u - nl;
         const double2 a = reinterpret_cast<const double2*>(Eb)[2 * v];
         const double2 c = reinterpret_cast<const double2*>(Eb)[2 * v + 1];
         reinterpret_cast<float4*>(E32b)[v] = make_float4((float)a.x, (float)a.y, (float)c.x, (float)c.y);
-      } else {
-        reinterpret_cast<float4*>(E32b)[u - nl] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     } else {
       if (u < nl) {
         const int e = (int)((unsigned)u % (unsigned)(n * n)), r = e / n;
         if (e - r * n <= r) L32b[u] = (float)Lb[u];
       }
-      else if (u - nl < ne) E32b[u - nl] = (float)Eb[u - nl];
-      else E32b[u - nl] = 0.f;
+      else E32b[u - nl] = (float)Eb[u - nl];
     }
   }
 }
@@ -582,14 +581,15 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
   const size_t xcount = (size_t)N * nm;
   hipError_t e = hipMemcpyAsync(p.b64, p.x, sizeof(double) * B * xcount, hipMemcpyDeviceToDevice, st);
   if (e != hipSuccess) return e;
-  // FP32 working copy of Psi (tril(L), E, fill-in region zeroed), in one or
-  // two launches: lcn elements of L, ecn of E, escn of E's whole storage.
+  // FP32 working copy of Psi (tril(L) and the N-1 coupling blocks of E; the
+  // factor writes the fill-in region), in one or two launches: lcn elements
+  // of L, ecn of E, escn the per-entry stride of E's storage.
   const size_t lc = (size_t)N * nn;
   const size_t ec = N > 1 ? (size_t)(N - 1) * nn : 0, es = N > 1 ? (size_t)nE * nn : 0;
   auto cast = [&](size_t lcn, size_t ecn, size_t escn, hipStream_t s) {
     const bool vec = lcn % 4 == 0 && ecn % 4 == 0 && escn % 4 == 0 && ((uintptr_t)p.L % 16) == 0 &&
                      ((uintptr_t)p.E % 16) == 0 && ((uintptr_t)p.L32 % 16) == 0 && ((uintptr_t)p.E32 % 16) == 0;
-    const size_t units = (lcn + escn) / (vec ? 4 : 1);
+    const size_t units = (lcn + ecn) / (vec ? 4 : 1);
     size_t blocks = (units + kThreads - 1) / kThreads;
     const size_t cap = (size_t)(2048 / B > 0 ? 2048 / B : 1);
     if (blocks > cap) blocks = cap;
@@ -622,14 +622,14 @@ hipError_t mixed_enqueue(const MixedPlan& p, hipStream_t st) {
       init(s);
       return hipGetLastError();
     };
-    if ((e = factor_with_forward<float>(p.L32, p.E32, p.d32, p.mask, p.active, B, N, n, m, p.f64_pivots, p.overlap,
+    if ((e = factor_with_forward<float>(p.L32, p.E32, p.d32, p.mask, p.active, B, N, n, m, p.prec, p.overlap,
                                         st, &first)) != hipSuccess)
       return e;
   } else {
     cast(lc, ec, es, st);
     init(st);
     for (int l = 0; l < Lv; ++l)
-      if ((e = factor_level_blocked<float>(p.L32, p.E32, p.mask, B, N, n, l, p.f64_pivots, st)) != hipSuccess)
+      if ((e = factor_level_blocked<float>(p.L32, p.E32, p.mask, B, N, n, l, p.prec, st)) != hipSuccess)
         return e;
   }
   const size_t rlds = mixed_residual_lds(n, m);

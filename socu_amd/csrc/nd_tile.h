@@ -113,6 +113,77 @@ __device__ __forceinline__ void wave_gemm16(typename Mfma<T>::acc_t& acc, const 
   }
 }
 
+// BF16 products with FP32 accumulation (v_mfma_f32_16x16x16_bf16): lane l
+// holds A[l%16][4(l/16) .. 4(l/16)+3] and B[4(l/16) .. 4(l/16)+3][l%16]
+// (the k order inside a group is the same for A and B, which is all the sum
+// needs); D has the FP32 16x16x4 layout.  Operands come from FP32 arrays and
+// are rounded to BF16 (nearest, ties to even) as they are read.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short f32_to_bf16(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);  // finite inputs: round to nearest even
+  return (short)(u >> 16);
+}
+
+__device__ __forceinline__ f32x4 mma_bf16(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// As wave_gemm<float> with the products on BF16 MFMA; K a multiple of 16.
+__device__ __forceinline__ void wave_gemm_bf16(f32x4 acc[2][2], const float* A0, int ai, int ak, const float* B0,
+                                               int bk, int bj, int K) {
+  const int l = lane_id();
+  const int i = l & 15, kq = 4 * (l >> 4);
+  for (int k = 0; k < K; k += 16) {
+    bf16x4 a0, a1, b0, b1;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int kk = k + kq + t;
+      a0[t] = f32_to_bf16(A0[i * ai + kk * ak]);
+      a1[t] = f32_to_bf16(A0[(i + 16) * ai + kk * ak]);
+      b0[t] = f32_to_bf16(B0[kk * bk + i * bj]);
+      b1[t] = f32_to_bf16(B0[kk * bk + (i + 16) * bj]);
+    }
+    acc[0][0] = mma_bf16(a0, b0, acc[0][0]);
+    acc[0][1] = mma_bf16(a0, b1, acc[0][1]);
+    acc[1][0] = mma_bf16(a1, b0, acc[1][0]);
+    acc[1][1] = mma_bf16(a1, b1, acc[1][1]);
+  }
+}
+
+// As wave_gemm16<float> with the products on BF16 MFMA; K a multiple of 16.
+__device__ __forceinline__ void wave_gemm16_bf16(f32x4& acc, const float* A0, int ai, int ak, const float* B0, int bk,
+                                                 int bj, int K) {
+  const int l = lane_id();
+  const int i = l & 15, kq = 4 * (l >> 4);
+  for (int k = 0; k < K; k += 16) {
+    bf16x4 a, b;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int kk = k + kq + t;
+      a[t] = f32_to_bf16(A0[i * ai + kk * ak]);
+      b[t] = f32_to_bf16(B0[kk * bk + i * bj]);
+    }
+    acc = mma_bf16(a, b, acc);
+  }
+}
+
+// Schur-update products in the storage precision, or (BF, float storage) on
+// BF16 MFMA.
+template <typename T, bool BF>
+__device__ __forceinline__ void schur_gemm(typename Mfma<T>::acc_t acc[2][2], const T* A0, int ai, int ak,
+                                           const T* B0, int bk, int bj, int K) {
+  if constexpr (BF) wave_gemm_bf16(acc, A0, ai, ak, B0, bk, bj, K);
+  else wave_gemm<T>(acc, A0, ai, ak, B0, bk, bj, K);
+}
+template <typename T, bool BF>
+__device__ __forceinline__ void schur_gemm16(typename Mfma<T>::acc_t& acc, const T* A0, int ai, int ak, const T* B0,
+                                             int bk, int bj, int K) {
+  if constexpr (BF) wave_gemm16_bf16(acc, A0, ai, ak, B0, bk, bj, K);
+  else wave_gemm16<T>(acc, A0, ai, ak, B0, bk, bj, K);
+}
+
 // Apply f(row, col, value) to every element of a 32x32 accumulator.
 template <typename T, typename F>
 __device__ __forceinline__ void acc_foreach(typename Mfma<T>::acc_t acc[2][2], F f) {

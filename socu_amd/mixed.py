@@ -8,8 +8,16 @@ README.md:274-285); this is an extension on the same data layout:
 * x (N,n,m) / (B,N,n,m), float64, holds b on entry and the solution on exit.
 
 The launch owns an FP32 working copy of (L, E), factors it with the same ND
-kernels (Schur updates on FP32 MFMA; with ``precision="float32"`` the diagonal
-blocks are factored and inverted in FP64 and rounded to FP32), then refines:
+kernels, then refines.  ``precision``:
+
+* ``"float32"`` (default): Schur updates on FP32 MFMA; the diagonal blocks are
+  factored and inverted in FP64 and rounded to FP32;
+* ``"bfloat16"``: as ``"float32"``, but the Schur-complement products (syrk
+  and the fill-in gemm) run on BF16 MFMA with FP32 accumulation, their
+  operands rounded from FP32 to BF16;
+* ``"float32_f32_pivots"``: everything in FP32.
+
+Refinement:
 d = solve32(r), x += d, r = b - Psi x in FP64, until ||r||_F <= tol ||b||_F
 per system or ``max_iter`` correction solves.  The convergence test runs on
 the device; with ``use_cuda_graph`` the whole call is one hipGraph launch.
@@ -26,7 +34,7 @@ from .block_tridiag_solver import _batched, _check_mask, _check_tensors, _ptr, _
 
 __all__ = ["create_cholesky_mixed_factor_and_solve_launch"]
 
-_PRECISIONS = {"float32": _lib.PREC_FP32, "float32_f32_pivots": _lib.PREC_FP32_F32PIV}
+_PRECISIONS = {"float32": _lib.PREC_FP32, "float32_f32_pivots": _lib.PREC_FP32_F32PIV, "bfloat16": _lib.PREC_BF16}
 
 
 class _MixedLaunch:

@@ -214,3 +214,52 @@ def test_direct_solve_entry_point_with_workspace(device):
                                        work.data_ptr(), st), "solve")
     torch.cuda.synchronize()
     assert _rel(x.cpu().numpy(), xr) < 1e-11
+
+
+@pytest.mark.parametrize("n,N,m,dtype", [(32, 37, 1, "float64"), (128, 24, 2, "float64"), (64, 50, 1, "float32"),
+                                         (10, 20, 3, "float64")])
+def test_factor_ignores_the_fill_in_region_on_entry(n, N, m, dtype, device):
+    """The reference needs E[N-1:] zeroed before each factor (it accumulates
+    F -= R Lc there); this factor writes each fill-in block once, so noise in
+    that region on entry must give the oracle's factor and solution."""
+    import torch
+    from socu_amd.block_tridiag_solver import create_cholesky_factor_and_solve_launch
+
+    tdt = getattr(torch, dtype)
+    L0, E0, b0 = _stack(n, N, m, 2, seed=31)
+    Lr, Er, xr = L0.copy(), E0.copy(), b0.copy()
+    nd.factor_and_solve(Lr, Er, xr)
+    En = E0.copy()
+    En[:, N - 1:] = np.random.RandomState(6).randn(*En[:, N - 1:].shape) * 1e6
+    L = torch.tensor(L0, dtype=tdt, device=device)
+    E = torch.tensor(En, dtype=tdt, device=device)
+    x = torch.tensor(b0, dtype=tdt, device=device)
+    create_cholesky_factor_and_solve_launch(L, E, x, dtype=tdt)()
+    tol = 1e-11 if dtype == "float64" else 2e-4
+    assert _rel(np.tril(L.cpu().numpy()), np.tril(Lr)) < tol
+    assert _rel(E.cpu().numpy(), Er) < tol
+    assert _rel(x.cpu().numpy(), xr) < tol
+
+
+def test_batched_factor_ignores_the_fill_in_region_on_entry(device):
+    import torch
+    from socu_amd.batched import create_batched_cholesky_launch
+
+    systems, refs = [], []
+    rs = np.random.RandomState(7)
+    for k, (n, N) in enumerate([(8, 33), (16, 20), (24, 47), (32, 64), (12, 5)]):
+        L0, E0, b0, _, _ = nd.readme_problem(n, N, 1, alpha=10.0, seed=50 + k)
+        Lr, Er, xr = L0.copy(), E0.copy(), b0.copy()
+        nd.factor_and_solve(Lr, Er, xr)
+        En = E0.copy()
+        En[N - 1:] = rs.randn(*En[N - 1:].shape) * 1e6
+        systems.append((torch.tensor(L0, device=device), torch.tensor(En, device=device),
+                        torch.tensor(b0, device=device)))
+        refs.append((Lr, Er, xr))
+    create_batched_cholesky_launch([s[0] for s in systems], [s[1] for s in systems], [s[2] for s in systems],
+                                   op="factor_and_solve", tune=False)()
+    torch.cuda.synchronize()
+    for (L, E, x), (Lr, Er, xr) in zip(systems, refs):
+        assert _rel(np.tril(L.cpu().numpy()), np.tril(Lr)) < 1e-11
+        assert _rel(E.cpu().numpy(), Er) < 1e-11
+        assert _rel(x.cpu().numpy(), xr) < 1e-11

@@ -14,11 +14,13 @@ mathematics fixes, independently of the code under test.
 * the C restatement against the numpy one;
 * planted mistakes (in the oracle, on the CPU) that these checks must catch.
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.linalg as la
 
-from oracle import nd
+from oracle import nd, nd_c
 
 
 # ---------------------------------------------------------------------------
@@ -289,3 +291,76 @@ def test_planted_mistakes_are_detected(mistake):
     Lg, Eg = L[None].copy(), E[None].copy()
     _factor_with("none", Lg, Eg)
     assert max(np.abs(np.tril(Lg[0]) - Lx).max(), np.abs(Eg[0] - Ex).max()) < 1e-10
+
+
+def test_round_bf16_matches_torch_bfloat16():
+    """The BF16 emulation's rounding (FP32, then nearest BF16, ties to even)
+    is torch's own float32 -> bfloat16 conversion, bit for bit, including
+    ties, powers of two and subnormal-range values."""
+    import torch
+
+    rs = np.random.RandomState(0)
+    x = np.concatenate([rs.randn(20000) * 10.0 ** rs.randint(-30, 30, 20000),
+                        # exact ties: 8 mantissa bits kept, then a half ulp
+                        np.float32(1 + np.arange(256) / 128.0 + 1 / 256.0).astype(np.float64),
+                        2.0 ** np.arange(-126, 127), [0.0, -0.0, 1e-40, -3e-39]])
+    got = nd.round_bf16(x)
+    want = torch.tensor(x, dtype=torch.float32).to(torch.bfloat16).double().numpy()
+    np.testing.assert_array_equal(got, want)
+
+
+def test_schur_operand_hook():
+    """factor(schur_operand=identity) is the plain factor; BF16 operands move
+    the factor by about a BF16 rounding (relative 1e-4 .. 1e-2), not more."""
+    L0, E0, _, _, _ = nd.readme_problem(12, 20, 1, alpha=16.0, seed=1)
+    La, Ea = L0.copy(), E0.copy()
+    nd.factor(La, Ea)
+    Lb, Eb = L0.copy(), E0.copy()
+    nd.factor(Lb, Eb, schur_operand=lambda a: a)
+    np.testing.assert_array_equal(np.tril(La), np.tril(Lb))
+    np.testing.assert_array_equal(Ea, Eb)
+    Lc, Ec = L0.copy(), E0.copy()
+    nd.factor(Lc, Ec, schur_operand=nd.round_bf16)
+    d = np.linalg.norm(np.tril(Lc) - np.tril(La)) / np.linalg.norm(np.tril(La))
+    assert 1e-5 < d < 1e-2
+
+
+def test_c_oracle_capi_shape_matches_the_product():
+    """The C oracle's ndo_cholesky_* entry points take the argument lists of
+    the product's socu_cholesky_* (parsed from include/socu_hip.h and from the
+    C source) and compute what its ndo_* functions compute."""
+    import re
+
+    from socu_amd import _lib as plib
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+    def params(text, name):
+        m = re.search(r"\b" + name + r"\s*\(([^)]*)\)", text)
+        assert m, name
+        return [re.sub(r"\s+", " ", p.strip()).rsplit(" ", 1)[0].replace(" *", "*") for p in m.group(1).split(",")]
+
+    hdr = re.sub(r"/\*.*?\*/", "", open(os.path.join(root, "include", "socu_hip.h")).read(), flags=re.S)
+    src = open(os.path.join(root, "oracle", "nd_oracle.c")).read()
+    for name in ("cholesky_factor", "cholesky_forward_substitution", "cholesky_backward_substitution",
+                 "cholesky_solve", "cholesky_factor_and_solve"):
+        assert params(hdr, "socu_" + name) == params(src, "ndo_" + name), name
+    lib = nd_c.capi()
+    n, N, m, B = 6, 13, 2, 2
+    L0 = np.stack([nd.readme_problem(n, N, m, alpha=10.0, seed=s)[0] for s in range(B)])
+    E0 = np.stack([nd.readme_problem(n, N, m, alpha=10.0, seed=s)[1] for s in range(B)])
+    b0 = np.stack([nd.readme_problem(n, N, m, alpha=10.0, seed=s)[2] for s in range(B)])
+    L, E, x = L0.copy(), E0.copy(), b0.copy()
+    assert lib.ndo_cholesky_factor_and_solve(L.ctypes.data, E.ctypes.data, x.ctypes.data, None, B, N, n, m,
+                                             plib.DTYPE_F64, None) == 0
+    Lr, Er, xr = L0.copy(), E0.copy(), b0.copy()
+    nd_c.factor_and_solve(Lr, Er, xr)
+    np.testing.assert_array_equal(x, xr)
+    y = b0.copy()
+    assert lib.ndo_cholesky_forward_substitution(L.ctypes.data, E.ctypes.data, y.ctypes.data, None, B, N, n, m,
+                                                 plib.DTYPE_F64, None, None) == 0
+    assert lib.ndo_cholesky_backward_substitution(L.ctypes.data, E.ctypes.data, y.ctypes.data, None, B, N, n, m,
+                                                  plib.DTYPE_F64, None, None) == 0
+    np.testing.assert_allclose(y, xr, rtol=1e-13, atol=0)
+    assert lib.ndo_cholesky_factor(L.ctypes.data, E.ctypes.data, None, B, N, n, plib.DTYPE_F32, None) == plib.EUNSUPPORTED
+    assert lib.ndo_cholesky_factor(None, None, None, 1, N, n, plib.DTYPE_F64, None) == plib.EINVAL
