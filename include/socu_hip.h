@@ -103,6 +103,21 @@ int socu_cholesky_backward_substitution(const void* L, const void* E, void* x, c
 int socu_cholesky_solve(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
                         int dtype, void* work, void* stream);
 
+/* socu_cholesky_solve without a workspace argument and without any device
+ * allocation: no counterpart in the reference; it is for callers that cannot
+ * allocate (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels and results as socu_cholesky_solve; the
+ * diagonal-block inverses go to a static device array of
+ * SOCU_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends
+ * (its address is looked up on the first call on a device).  Returns
+ * SOCU_EUNSUPPORTED when socu_solve_workspace_size(B, N, n, dtype) exceeds
+ * it (64 MiB: e.g. B*N <= 512 FP64 blocks of n = 128).  Every call on one
+ * GPU uses the same array, so calls that overlap on one GPU (other streams,
+ * other threads) race; calls in order on one stream are safe. */
+#define SOCU_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int socu_cholesky_solve_noalloc(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N, int n,
+                                int m, int dtype, void* stream);
+
 /* factor, forward, backward, block_tridiag_solver.py:1124-1228.  The
  * factor leaves the transposed inverse of each diagonal factor block in the
  * strict upper triangle of L[b, i] (the reference leaves that triangle

@@ -278,6 +278,38 @@ int sweep_status(int* ws, int* hstatus, int* persistent, int* expired, void* str
   return SOCU_OK;
 }
 
+// Scratch of socu_cholesky_solve_noalloc: the diagonal-block inverses its
+// sweeps read (socu_cholesky_solve's `work`).  Every GPU the library runs on
+// holds one copy from module load until the process ends; calls on one GPU
+// share it, so calls that overlap on one GPU race.  Host code never passes
+// `scratch` itself (its host-side name is not a device address): it uses the
+// per-device address that hipGetSymbolAddress returns.
+constexpr size_t kNoAllocScratchBytes = SOCU_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call on
+// that device (with it current) and kept.
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(SOCU_EUNSUPPORTED, "device %d beyond %d devices", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_addr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress (no-allocation scratch)");
+    if (p == nullptr) return fail(SOCU_EHIP, "hipGetSymbolAddress returned NULL for the no-allocation scratch");
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return SOCU_OK;
+}
+
 // An execution whose persistent sweep had a wait expire left NaN in its
 // solution and raised the host-visible word; the next execution reports it
 // (and clears the word) instead of running.
@@ -359,6 +391,23 @@ int socu_cholesky_solve(const void* L, const void* E, void* x, const unsigned ch
                         int dtype, void* work, void* stream) {
   return dispatch(SOCU_OP_SOLVE, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream,
                   work);
+}
+
+int socu_cholesky_solve_noalloc(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N, int n,
+                                int m, int dtype, void* stream) {
+  int rc = check_dims(L, E, x, true, B, N, n, m, dtype);
+  if (rc) return rc;
+  if (B == 0) return SOCU_OK;
+  const size_t need = inverse_ws_bytes(B, N, n, dtype);
+  if (need > kNoAllocScratchBytes)
+    return fail(SOCU_EUNSUPPORTED,
+                "socu_cholesky_solve_noalloc: B=%d N=%d n=%d needs %zu bytes of diagonal-block inverses, more than its "
+                "%zu-byte scratch; use socu_cholesky_solve with a caller-provided workspace",
+                B, N, n, need, kNoAllocScratchBytes);
+  void* ws = nullptr;
+  rc = noalloc_scratch(&ws);
+  if (rc) return rc;
+  return dispatch(SOCU_OP_SOLVE, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream, ws);
 }
 
 int socu_cholesky_factor_and_solve(void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
