@@ -103,6 +103,18 @@ int socu_cholesky_backward_substitution(const void* L, const void* E, void* x, c
 int socu_cholesky_solve(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
                         int dtype, void* work, void* stream);
 
+/* forward then backward sweep with a factor made by this library's
+ * socu_cholesky_factor or socu_cholesky_factor_and_solve, which leave the
+ * transposed inverse of each diagonal factor block in the strict upper
+ * triangle of L[b, i]: the sweeps read it there instead of forming it from
+ * tril(L), so no workspace is needed and the inverse launch is saved (no
+ * counterpart in the reference, whose L upper triangle is unspecified; a
+ * factor made any other way needs socu_cholesky_solve).  For callers that
+ * keep the factor they made, e.g. the backward pass of a differentiable
+ * solve. */
+int socu_cholesky_solve_with_inverses(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N,
+                                      int n, int m, int dtype, void* stream);
+
 /* socu_cholesky_solve without a workspace argument and without any device
  * allocation: no counterpart in the reference; it is for callers that cannot
  * allocate (code under hipGraph capture, tight serving loops, harnesses that

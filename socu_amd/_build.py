@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsocu_hip.so")
 
-SOURCES = ["nd_blocked.hip", "nd_refine.hip", "nd_batched.hip", "nd_adjoint.hip", "nd_sweep.hip", "capi.hip"]
+SOURCES = ["nd_blocked.hip", "nd_large.hip", "nd_refine.hip", "nd_batched.hip", "nd_adjoint.hip", "nd_sweep.hip",
+           "capi.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

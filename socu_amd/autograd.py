@@ -75,10 +75,12 @@ class BlockTridiagSolve(torch.autograd.Function):
         lib = _lib.load()
         dev = x.device
         bbar = xbar.reshape(B, N, n, m).to(dtype=x.dtype).contiguous().clone()
-        work = torch.empty(lib.socu_solve_workspace_size(B, N, n, code), dtype=torch.uint8, device=dev)
         with torch.cuda.device(dev):
-            _lib.check(lib.socu_cholesky_solve(Lw.data_ptr(), Ew.data_ptr(), bbar.data_ptr(), None, B, N, n, m, code,
-                                               work.data_ptr(), _stream(dev)), "block_tridiag_solve backward (solve)")
+            # Lw is this library's factor: its diagonal-block inverses are in
+            # place, so the transpose solve needs no inverse launch
+            _lib.check(lib.socu_cholesky_solve_with_inverses(Lw.data_ptr(), Ew.data_ptr(), bbar.data_ptr(), None, B, N,
+                                                             n, m, code, _stream(dev)),
+                       "block_tridiag_solve backward (solve)")
             dL = dE = None
             if ctx.needs_input_grad[0] or ctx.needs_input_grad[1]:
                 dL = torch.empty_like(Lw)

@@ -43,8 +43,8 @@ int check_dims(const void* L, const void* E, const void* x, bool need_x, int B, 
   if (B < 0) return fail(SOCU_EINVAL, "batch dimension must be >= 0 (got %d)", B);
   if (N < 1) return fail(SOCU_EINVAL, "horizon N must be >= 1 (got %d)", N);
   if (n < 1) return fail(SOCU_EINVAL, "block size n must be >= 1 (got %d)", n);
-  if (n > socu::kMaxN)
-    return fail(SOCU_EUNSUPPORTED, "block size n=%d exceeds the supported maximum %d", n, socu::kMaxN);
+  if (n > socu::kMaxNLarge)
+    return fail(SOCU_EUNSUPPORTED, "block size n=%d exceeds the supported maximum %d", n, socu::kMaxNLarge);
   if (need_x && m < 1) return fail(SOCU_EINVAL, "number of right-hand sides m must be >= 1 (got %d)", m);
   if (B > 0) {
     if (L == nullptr) return fail(SOCU_EINVAL, "L is NULL");
@@ -119,11 +119,13 @@ int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N,
   T* Et = static_cast<T*>(E);
   T* xt = static_cast<T*>(x);
   int rc = SOCU_OK;
-  if (op == SOCU_OP_FORWARD || op == SOCU_OP_SOLVE || op == SOCU_OP_BACKWARD) {
+  if ((op == SOCU_OP_FORWARD || op == SOCU_OP_SOLVE || op == SOCU_OP_BACKWARD) && inv != nullptr) {
     // The sweeps multiply by the inverses of the diagonal factor blocks;
     // they are formed here from tril(L), so the factor may come from
     // anywhere (only tril(L) is read, as in the reference), and the solve
-    // reads them from the workspace in place of L.
+    // reads them from the workspace in place of L.  (Without a workspace
+    // the caller vouches that L holds them already, as this library's
+    // factor leaves them.)
     T* W = static_cast<T*>(inv);
     const hipError_t e = socu::trtri_blocked<T>(Lt, W, mask, B, N, n, st);
     if (e != hipSuccess) return hip_fail(e, "diagonal-block inverse launch");
@@ -160,6 +162,7 @@ int ensure_init() {
   if (e == hipSuccess) e = socu::init_refine_kernels();
   if (e == hipSuccess) e = socu::init_batched_kernels();
   if (e == hipSuccess) e = socu::init_sweep_kernels();
+  if (e == hipSuccess) e = socu::init_large_kernels();
   if (e != hipSuccess) return hip_fail(e, "kernel set-up (hipFuncSetAttribute)");
   state = 1;
   return SOCU_OK;
@@ -175,12 +178,13 @@ size_t inverse_ws_bytes(int B, int N, int n, int dtype) {
 bool op_needs_inverse(int op) { return op == SOCU_OP_FORWARD || op == SOCU_OP_SOLVE || op == SOCU_OP_BACKWARD; }
 
 int dispatch(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m, int dtype,
-             void* stream, void* inv, const SweepWs& sw = SweepWs(), const socu::SweepOverlap* ov = nullptr) {
+             void* stream, void* inv, const SweepWs& sw = SweepWs(), const socu::SweepOverlap* ov = nullptr,
+             bool stored_inverse = false) {
   const bool need_x = op != SOCU_OP_FACTOR;
   int rc = check_dims(L, E, x, need_x, B, N, n, m, dtype);
   if (rc) return rc;
   if (B == 0) return SOCU_OK;
-  if (op_needs_inverse(op) && inv == nullptr)
+  if (op_needs_inverse(op) && inv == nullptr && !stored_inverse)
     return fail(SOCU_EINVAL, "work is NULL: substitutions need socu_solve_workspace_size() bytes of device workspace");
   rc = ensure_init();
   if (rc) return rc;
@@ -210,6 +214,22 @@ void kt_begin(int cls, hipStream_t st) {
   g_timer.cls.push_back(cls);
   g_timer.ev.push_back(a);
   g_timer.ev.push_back(b);
+}
+
+bool kt_kernel_events(int cls, hipEvent_t* start, hipEvent_t* stop) {
+  if (!g_timer.active) return false;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess) return false;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return false;
+  }
+  g_timer.cls.push_back(cls);
+  g_timer.ev.push_back(a);
+  g_timer.ev.push_back(b);
+  *start = a;
+  *stop = b;
+  return true;
 }
 
 void kt_end(int cls, hipStream_t st) {
@@ -335,7 +355,7 @@ long long socu_calculate_off_diag_storage_len(int N) {
   return socu::off_diag_storage_len(N);
 }
 
-int socu_max_block_size(void) { return socu::kMaxN; }
+int socu_max_block_size(void) { return socu::kMaxNLarge; }
 
 int socu_set_sweep_spin_limit(int polls) {
   socu::set_sweep_spin_limit(polls);
@@ -357,6 +377,8 @@ int socu_cholesky_factor_f32(void* L, void* E, const unsigned char* mask, int B,
                              void* stream) {
   int rc = check_dims(L, E, nullptr, false, B, N, n, 1, SOCU_DTYPE_F32);
   if (rc) return rc;
+  if (n > socu::kMaxN)
+    return fail(SOCU_EUNSUPPORTED, "socu_cholesky_factor_f32: block size n=%d exceeds %d", n, socu::kMaxN);
   const int prec = precision == SOCU_PREC_FP32 ? socu::kPrecF64Pivots
                    : precision == SOCU_PREC_BF16 ? socu::kPrecF64Pivots | socu::kPrecBf16Schur
                    : precision == SOCU_PREC_FP32_F32PIV ? 0
@@ -391,6 +413,12 @@ int socu_cholesky_solve(const void* L, const void* E, void* x, const unsigned ch
                         int dtype, void* work, void* stream) {
   return dispatch(SOCU_OP_SOLVE, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream,
                   work);
+}
+
+int socu_cholesky_solve_with_inverses(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N,
+                                      int n, int m, int dtype, void* stream) {
+  return dispatch(SOCU_OP_SOLVE, const_cast<void*>(L), const_cast<void*>(E), x, mask, B, N, n, m, dtype, stream,
+                  nullptr, SweepWs(), nullptr, true);
 }
 
 int socu_cholesky_solve_noalloc(const void* L, const void* E, void* x, const unsigned char* mask, int B, int N, int n,
@@ -585,6 +613,8 @@ int socu_mixed_create(socu_mixed** out, const void* L, const void* E, void* x, c
     return fail(SOCU_EINVAL, "unknown factor precision %d", precision);
   if (!(tol > 0.0)) return fail(SOCU_EINVAL, "tolerance must be positive");
   if (max_iter < 1 || max_iter > 100) return fail(SOCU_EINVAL, "max_iter must be in [1, 100] (got %d)", max_iter);
+  if (n > socu::kMaxN)
+    return fail(SOCU_EUNSUPPORTED, "mixed-precision solve: block size n=%d exceeds %d", n, socu::kMaxN);
   if (!socu::mixed_fits(n, m))
     return fail(SOCU_EUNSUPPORTED, "n*m = %d too large for the residual kernel (max 4608)", n * m);
   if (B > 0) {

@@ -23,6 +23,8 @@
 //                block_tridiag_solver.py:425-736).
 #include <stdlib.h>
 
+#include <hip/hip_ext.h>
+
 #include "nd_common.h"
 #include "nd_internal.h"
 #include "nd_tile.h"
@@ -1386,18 +1388,22 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
   const Level lv = level_info(N, lidx);
   if (lv.dim <= 0 || B <= 0) return hipSuccess;
   const size_t lds_trsm = trsm_blk_lds<T, NB>();
-  kt_begin(kKcPotrf, st);
   const bool bf16 = (prec & kPrecBf16Schur) != 0 && sizeof(T) == 4;
-  if ((prec & kPrecF64Pivots) != 0 && sizeof(T) == 4) {
-    const size_t lds_potrf = potrf_blk_lds<double, NB>();
-    hipLaunchKernelGGL((k_potrf_blk<T, NB, NF, double>), dim3(lv.dim, B), dim3(potrf_threads(NB)), lds_potrf, st, L, mask, N,
-                       n, lv.s);
-  } else {
-    const size_t lds_potrf = potrf_blk_lds<T, NB>();
-    hipLaunchKernelGGL((k_potrf_blk<T, NB, NF, T>), dim3(lv.dim, B), dim3(potrf_threads(NB)), lds_potrf, st, L, mask, N, n,
-                       lv.s);
+  {
+    // under socu_profile_op the launch itself records its start and end
+    // (events of hipExtLaunchKernel: the kernel's own execution interval)
+    auto k = k_potrf_blk<T, NB, NF, T>;
+    size_t lds_potrf = potrf_blk_lds<T, NB>();
+    if ((prec & kPrecF64Pivots) != 0 && sizeof(T) == 4) {
+      k = k_potrf_blk<T, NB, NF, double>;
+      lds_potrf = potrf_blk_lds<double, NB>();
+    }
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (kt_kernel_events(kKcPotrf, &ea, &eb))
+      hipExtLaunchKernelGGL(k, dim3(lv.dim, B), dim3(potrf_threads(NB)), lds_potrf, st, ea, eb, 0, L, mask, N, n, lv.s);
+    else
+      hipLaunchKernelGGL(k, dim3(lv.dim, B), dim3(potrf_threads(NB)), lds_potrf, st, L, mask, N, n, lv.s);
   }
-  kt_end(kKcPotrf, st);
   if (after_potrf != nullptr) {
     const hipError_t e = hipEventRecord(after_potrf, st);
     if (e != hipSuccess) return e;
@@ -1452,10 +1458,47 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
   return hipGetLastError();
 }
 
+// One factor level for kMaxN < n <= kMaxNLarge: the tiled potrf and trsm of
+// nd_large.hip, then the Schur updates of k_schur_blk (which walks any n).
+template <typename T>
+static hipError_t factor_level_large(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, hipStream_t st,
+                                     hipEvent_t after_potrf, hipEvent_t after_trsm, hipEvent_t before_trsm) {
+  const int nE = (int)off_diag_storage_len(N);
+  const int Lv = recursive_iterations(N);
+  const Level lv = level_info(N, lidx);
+  if (lv.dim <= 0 || B <= 0) return hipSuccess;
+  kt_begin(kKcPotrf, st);
+  hipError_t e = potrf_large<T>(L, mask, B, N, n, lidx, st);
+  kt_end(kKcPotrf, st);
+  if (e == hipSuccess && after_potrf != nullptr) e = hipEventRecord(after_potrf, st);
+  if (e != hipSuccess || lidx == Lv - 1) return e;
+  if (before_trsm != nullptr && (e = hipStreamWaitEvent(st, before_trsm, 0)) != hipSuccess) return e;
+  kt_begin(kKcTrsm, st);
+  e = trsm_large<T>(L, E, mask, B, N, n, lidx, st);
+  kt_end(kKcTrsm, st);
+  if (e == hipSuccess && after_trsm != nullptr) e = hipEventRecord(after_trsm, st);
+  if (e != hipSuccess) return e;
+  const int ntile = (n + kSTile - 1) / kSTile;
+  const int ntri = ntile * (ntile + 1) / 2;
+  const int nT = N / (2 * lv.s);
+  int nF = 0;
+  for (int t = 1; t < lv.dim; ++t)
+    if (lv.s - 1 + 2 * lv.s * t + lv.s < N) ++nF;
+  const int tasks = nT * ntri + nF * ntile * ntile;
+  if (tasks > 0) {
+    kt_begin(kKcSchur, st);
+    hipLaunchKernelGGL((k_schur_blk<T, 0, false>), dim3(tasks, B), dim3(kThreads), 0, st, L, E, mask, N, n, nE, lv.s,
+                       lv.off, lv.next_off, nT, ntile);
+    kt_end(kKcSchur, st);
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, int prec,
                                 hipStream_t st, hipEvent_t after_potrf, hipEvent_t after_trsm,
                                 hipEvent_t before_trsm) {
+  if (n > kMaxN) return factor_level_large<T>(L, E, mask, B, N, n, lidx, st, after_potrf, after_trsm, before_trsm);
   const int fp = prec;
   switch (n) {
     case 32: return factor_level_nb<T, 1, 32>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
@@ -1485,6 +1528,12 @@ static hipError_t trtri_nb(const T* L, T* W, const uint8_t* mask, int B, int N, 
 
 template <typename T>
 hipError_t trtri_blocked(const T* L, T* W, const uint8_t* mask, int B, int N, int n, hipStream_t st) {
+  if (n > kMaxN) {
+    kt_begin(kKcForward, st);
+    const hipError_t e = trtri_large<T>(L, W, mask, B, N, n, st);
+    kt_end(kKcForward, st);
+    return e;
+  }
   switch (n) {
     case 32: return trtri_nb<T, 1, 32>(L, W, mask, B, N, n, st);
     case 64: return trtri_nb<T, 2, 64>(L, W, mask, B, N, n, st);
@@ -1526,6 +1575,14 @@ static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* m
 template <typename T>
 static hipError_t forward_step_parts(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
                               int step, hipStream_t st, bool a, bool u) {
+  if (n > kMaxN) {
+    hipError_t e = hipSuccess;
+    kt_begin(kKcForward, st);
+    if (u && step >= 0) e = update_large<T>(E, x, mask, B, N, n, m, step, false, st);
+    if (e == hipSuccess && a) e = apply_large<T>(L, x, mask, B, N, n, m, step + 1, false, st);
+    kt_end(kKcForward, st);
+    return e;
+  }
   switch (n) {
     case 32: return forward_step_nb<T, 1, 32>(L, E, x, mask, B, N, n, m, step, st, a, u);
     case 64: return forward_step_nb<T, 2, 64>(L, E, x, mask, B, N, n, m, step, st, a, u);
@@ -1581,6 +1638,15 @@ static hipError_t backward_level_nb(const T* L, const T* E, T* x, const uint8_t*
 template <typename T>
 hipError_t backward_level_blocked(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m,
                                   int lidx, hipStream_t st) {
+  if (n > kMaxN) {
+    const Level lv = level_info(N, lidx);
+    hipError_t e = hipSuccess;
+    kt_begin(kKcBackward, st);
+    if (!(lv.dim == 1 && lv.s - 1 + lv.s >= N)) e = update_large<T>(E, x, mask, B, N, n, m, lidx, true, st);
+    if (e == hipSuccess) e = apply_large<T>(L, x, mask, B, N, n, m, lidx, true, st);
+    kt_end(kKcBackward, st);
+    return e;
+  }
   switch (n) {
     case 32: return backward_level_nb<T, 1, 32>(L, E, x, mask, B, N, n, m, lidx, st);
     case 64: return backward_level_nb<T, 2, 64>(L, E, x, mask, B, N, n, m, lidx, st);

@@ -24,6 +24,11 @@ constexpr int kThreads = 256;
 // 128x129 fp64 block is 129 KiB of the 160 KiB of LDS per workgroup).
 constexpr int kMaxN = 128;
 
+// Largest block size n of the tiled kernels for larger blocks (the block in
+// HBM, 32 x 32 tiles through LDS; a staged 32-wide chunk of a 256-wide
+// block is 66 KiB of FP64 LDS).
+constexpr int kMaxNLarge = 256;
+
 // Column chunk of the right-hand sides processed by one workgroup.
 constexpr int kRhsChunk = 16;
 

@@ -25,6 +25,9 @@ enum KernelClass {
 // No-ops unless socu_profile_op is running on this thread.
 void kt_begin(int cls, hipStream_t st);
 void kt_end(int cls, hipStream_t st);
+// Under socu_profile_op: a begin/end event pair of class `cls` for a launch
+// that records them itself (hipExtLaunchKernel), and true; else false.
+bool kt_kernel_events(int cls, hipEvent_t* start, hipEvent_t* stop);
 
 // One-time per-process kernel set-up (LDS limits above 64 KiB).  Must run
 // before any stream capture.
@@ -40,7 +43,8 @@ constexpr int kPrecF64Pivots = 1;
 constexpr int kPrecBf16Schur = 2;
 
 // Kernels of one ND factor level (potrf, then trsm and Schur updates unless it
-// is the last level), for block size n <= kMaxN; prec: kPrec* flags.
+// is the last level), for block size n <= kMaxNLarge; prec: kPrec* flags
+// (n <= kMaxN only).
 // after_potrf / after_trsm (optional) are recorded on st after the level's
 // potrf and trsm launches (the forward sweep can follow them on another
 // stream); st waits for before_trsm (optional) between potrf and trsm.
@@ -48,6 +52,25 @@ template <typename T>
 hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, int prec,
                                 hipStream_t st, hipEvent_t after_potrf = nullptr, hipEvent_t after_trsm = nullptr,
                                 hipEvent_t before_trsm = nullptr);
+
+// Per-node kernels for kMaxN < n <= kMaxNLarge (nd_large.hip): potrf with the
+// transposed inverse in the strict upper triangle; trsm of level lidx's R and
+// Lc; the forward (U^T x) or backward (U x) apply with U = Linv^T as stored,
+// for level lidx's nodes; the sweep updates of level lidx; the inverse of a
+// factor from tril(L) alone into W.
+hipError_t init_large_kernels();
+template <typename T>
+hipError_t potrf_large(T* L, const uint8_t* mask, int B, int N, int n, int lidx, hipStream_t st);
+template <typename T>
+hipError_t trsm_large(const T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, hipStream_t st);
+template <typename T>
+hipError_t apply_large(const T* L, T* x, const uint8_t* mask, int B, int N, int n, int m, int lidx, bool bwd,
+                       hipStream_t st);
+template <typename T>
+hipError_t update_large(const T* E, T* x, const uint8_t* mask, int B, int N, int n, int m, int lidx, bool bwd,
+                        hipStream_t st);
+template <typename T>
+hipError_t trtri_large(const T* L, T* W, const uint8_t* mask, int B, int N, int n, hipStream_t st);
 
 // W <- tril(L) with the transposed inverse of each diagonal block in its
 // strict upper triangle (the layout the factor leaves in L), from tril(L)

@@ -563,10 +563,10 @@ hipError_t init_sweep_kernels() {
 // FP64 (n=64 FP32: 275 vs 133 us; n=128 FP64: 379 vs 339 us): a dependency
 // hop between workgroups costs about as much as a dependent kernel launch.
 bool sweep_persistent_enabled(int n, int dsize) {
+  if (n > kMaxN) return false;  // its node tasks hold whole blocks in LDS
   const char* v = getenv("SOCU_SWEEP");
   if (v != nullptr && strcmp(v, "level") == 0) return false;
   if (v != nullptr && strcmp(v, "persistent") == 0) return true;
-  (void)n;
   (void)dsize;
   return true;  // faster than the level kernels for every block size measured (see sweep_lmin)
 }

@@ -152,6 +152,13 @@ def _substitute(op, Lf, Ef, rhs):
     code = _code(Lf)
     Lc, Ec = Lf.contiguous(), Ef.contiguous()
     dev = x.device
+    if op == "solve_own":
+        # a factor made by this library's factor: its diagonal-block
+        # inverses are in place, no workspace or inverse launch needed
+        with torch.cuda.device(dev):
+            _lib.check(lib.socu_cholesky_solve_with_inverses(Lc.data_ptr(), Ec.data_ptr(), x.data_ptr(), None, B, N,
+                                                             n, m, code, _stream(dev)), "cholesky_solve")
+        return x
     work = torch.empty(lib.socu_solve_workspace_size(B, N, n, code), dtype=torch.uint8, device=dev)
     fn = {"forward": lib.socu_cholesky_forward_substitution, "backward": lib.socu_cholesky_backward_substitution,
           "solve": lib.socu_cholesky_solve}[op]
@@ -254,7 +261,7 @@ class _FactorAndSolve(torch.autograd.Function):
     def backward(ctx, gL, gE, gx):
         Lf, Ef, x = ctx.saved_tensors
         B, N, n, m = _dims(Lf, x)
-        bbar = _substitute("solve", Lf, Ef, gx)
+        bbar = _substitute("solve_own", Lf, Ef, gx)
         dL = dE = None
         if (ctx.needs_input_grad[0] or ctx.needs_input_grad[1]) and B > 0:
             lib = _lib.load()

@@ -263,3 +263,33 @@ def test_batched_factor_ignores_the_fill_in_region_on_entry(device):
         assert _rel(np.tril(L.cpu().numpy()), np.tril(Lr)) < 1e-11
         assert _rel(E.cpu().numpy(), Er) < 1e-11
         assert _rel(x.cpu().numpy(), xr) < 1e-11
+
+
+@pytest.mark.parametrize("n,N,m,dtype", [(32, 40, 1, "float64"), (96, 17, 3, "float32"), (200, 9, 2, "float64")])
+def test_solve_with_stored_inverses_equals_solve(n, N, m, dtype, device):
+    """socu_cholesky_solve_with_inverses on this library's own factor gives
+    socu_cholesky_solve's x to rounding: the inverses it reads in place (made
+    by the factor) and the ones the solve forms from tril(L) come from
+    different kernels, so they may differ in the last bits."""
+    import torch
+    from socu_amd import _lib
+
+    tdt = getattr(torch, dtype)
+    code = _lib.DTYPE_F64 if dtype == "float64" else _lib.DTYPE_F32
+    lib = _lib.load()
+    B = 2
+    L0, E0, b0 = _stack(n, N, m, B, seed=4)
+    L = torch.tensor(L0, dtype=tdt, device=device)
+    E = torch.tensor(E0, dtype=tdt, device=device)
+    st = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    _lib.check(lib.socu_cholesky_factor(L.data_ptr(), E.data_ptr(), None, B, N, n, code, st), "factor")
+    x1 = torch.tensor(b0, dtype=tdt, device=device)
+    x2 = x1.clone()
+    work = torch.empty(lib.socu_solve_workspace_size(B, N, n, code), dtype=torch.uint8, device=device)
+    _lib.check(lib.socu_cholesky_solve(L.data_ptr(), E.data_ptr(), x1.data_ptr(), None, B, N, n, m, code,
+                                       work.data_ptr(), st), "solve")
+    _lib.check(lib.socu_cholesky_solve_with_inverses(L.data_ptr(), E.data_ptr(), x2.data_ptr(), None, B, N, n, m,
+                                                     code, st), "solve_with_inverses")
+    torch.cuda.synchronize(device)
+    a, b = x1.cpu().numpy(), x2.cpu().numpy()
+    assert np.linalg.norm(b - a) / np.linalg.norm(a) < (1e-13 if dtype == "float64" else 1e-5)
