@@ -771,17 +771,40 @@ __device__ __forceinline__ void products64(typename Mfma<T>::acc_t acc[2][2], bo
   }
 }
 
+// Workgroups are handed to the eight XCDs round-robin (XCD = workgroup index
+// mod 8, gridDim.x a multiple of 8), and each XCD has its own L2.  Tasks are
+// numbered group-major -- group g: the output tiles of target g, then those
+// of fill-in g, which share E blocks 2g+1 (and read 2g, 2g+2) -- and XCD x
+// takes the contiguous range [x * per, (x + 1) * per) of that order, so the
+// tiles that re-read one operand block run on one XCD at about the same
+// time.  Slots past the last task (and the missing fill-in of the last
+// group) return at once.
+__host__ __device__ constexpr int schur_blk_groups(int nT, int nF) { return nT > nF ? nT : nF; }
+__host__ __device__ constexpr int schur_blk_per_xcd(int nT, int nF, int ntile) {
+  return (schur_blk_groups(nT, nF) * (ntile * (ntile + 1) / 2 + ntile * ntile) + 7) / 8;
+}
+
 template <typename T, int NF, bool BF = false>
 __global__ __launch_bounds__(kThreads) void k_schur_blk(T* __restrict__ L, T* __restrict__ E,
                                                        const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
-                                                       int off, int next_off, int nT, int ntile) {
+                                                       int off, int next_off, int nT, int nF, int ntile) {
   const int n = NF ? NF : n_;
   const int b = blockIdx.y;
   if (masked_out(mask, b)) return;
   __shared__ T As[kSK * kSP];
   __shared__ T Bs[kSK * kSP];
   const int ntri = ntile * (ntile + 1) / 2;
-  int task = blockIdx.x;
+  const int gs = ntri + ntile * ntile;
+  int task;
+  bool target;
+  {
+    const int per = schur_blk_per_xcd(nT, nF, ntile);
+    const int lg = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    const int g = lg / gs, q = lg - g * gs;
+    target = q < ntri;
+    if (target ? g >= nT : g >= nF) return;
+    task = target ? g * ntri + q : nT * ntri + g * ntile * ntile + (q - ntri);
+  }
   const size_t nn = (size_t)n * n;
   T* Eb = E + (size_t)b * nE * nn;
   const int wave = wave_id();
@@ -790,7 +813,7 @@ __global__ __launch_bounds__(kThreads) void k_schur_blk(T* __restrict__ L, T* __
   T* C;
   int tr, tc;
   bool active = true;
-  if (task < nT * ntri) {
+  if (target) {
     const int u = task / ntri;
     int q = task - u * ntri;
     tr = 0;
@@ -821,7 +844,7 @@ __global__ __launch_bounds__(kThreads) void k_schur_blk(T* __restrict__ L, T* __
   }
   if (!active) return;
   const int rb = tr * kSTile + 32 * wi, cb = tc * kSTile + 32 * wj;
-  if (blockIdx.x < nT * ntri) {
+  if (target) {
     acc_foreach<T>(acc, [&](int r, int c, T v) {
       const int rr = rb + r, cc = cb + c;
       if (rr < n && cc < n) C[(size_t)rr * n + cc] -= v;
@@ -1450,8 +1473,8 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
       auto k = k_schur_blk<T, NF, false>;
       if constexpr (sizeof(T) == 4)
         if (bf16) k = k_schur_blk<T, NF, true>;
-      hipLaunchKernelGGL(k, dim3(tasks, B), dim3(kThreads), 0, st, L, E, mask, N, n, nE, lv.s, lv.off, lv.next_off, nT,
-                         ntile);
+      hipLaunchKernelGGL(k, dim3(8 * schur_blk_per_xcd(nT, nF, ntile), B), dim3(kThreads), 0, st, L, E, mask, N, n, nE,
+                         lv.s, lv.off, lv.next_off, nT, nF, ntile);
     }
     kt_end(kKcSchur, st);
   }
@@ -1487,8 +1510,8 @@ static hipError_t factor_level_large(T* L, T* E, const uint8_t* mask, int B, int
   const int tasks = nT * ntri + nF * ntile * ntile;
   if (tasks > 0) {
     kt_begin(kKcSchur, st);
-    hipLaunchKernelGGL((k_schur_blk<T, 0, false>), dim3(tasks, B), dim3(kThreads), 0, st, L, E, mask, N, n, nE, lv.s,
-                       lv.off, lv.next_off, nT, ntile);
+    hipLaunchKernelGGL((k_schur_blk<T, 0, false>), dim3(8 * schur_blk_per_xcd(nT, nF, ntile), B), dim3(kThreads), 0, st,
+                       L, E, mask, N, n, nE, lv.s, lv.off, lv.next_off, nT, nF, ntile);
     kt_end(kKcSchur, st);
   }
   return hipGetLastError();

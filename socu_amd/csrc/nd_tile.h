@@ -207,7 +207,8 @@ __device__ __forceinline__ void acc16_foreach(typename Mfma<T>::acc_t& acc, F f)
 // S: tile in LDS, row-major with `pitch`; its lower triangle is the input.
 // On return the lower triangle holds L, the strict upper triangle holds the
 // transposed inverse (S[c][q] = Linv[q][c] for q > c) and dinv[r] = 1/L[r][r].
-// col: 128 entries of wave-private LDS (two buffers of two columns).
+// col: 128 entries of wave-private LDS (two buffers of two columns; its
+// first 64 entries are overwritten at the end).
 // ut: a 32 x 33 LDS tile that receives Linv transposed in full (ut[r][c] =
 // Linv[c][r], zeros and diagonal included).
 //
@@ -238,11 +239,21 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col, 
   const int r = lane & 31;
   const bool rowlane = lane < 32;
   T u[32];
-  T rinv = T(1);
+  {
+    // Row lanes take row r as it is, strict upper entries included: those
+    // entries only ever meet the multipliers of their own lane (see below)
+    // and are not stored.  The reads are unconditional (in bounds) and
+    // pinned, so that they are all in flight at once; inverse lanes then
+    // replace theirs by the unit vector e_r.  (Selects on the reads let the
+    // compiler turn every read into a branch with a wait of its own.)
 #pragma unroll
-  for (int c = 0; c < 32; ++c) {
-    const T v = S[r * pitch + c];  // in-bounds; upper entries are discarded
-    u[c] = rowlane ? ((c <= r) ? v : T(0)) : ((c == r) ? T(1) : T(0));
+    for (int c = 0; c < 32; ++c) u[c] = S[r * pitch + c];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) asm volatile("" : "+v"(u[c]));
+    if (!rowlane) {
+#pragma unroll
+      for (int c = 0; c < 32; ++c) u[c] = (c == r) ? T(1) : T(0);
+    }
   }
   // one base address in a vector register: every column access below is
   // that base plus an immediate offset
@@ -350,7 +361,6 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col, 
     const bool on1 = rk == k, on2 = rk == k + 1;
     u[k] = on1 ? pk1 : lk1;
     u[k + 1] = on2 ? pk2 : lk2;
-    rinv = on1 ? inv1 : (on2 ? inv2 : rinv);
     // Materialise the registers every step: otherwise the optimiser sinks
     // the updates to their last use and keeps every broadcast value alive.
 #pragma unroll
@@ -363,15 +373,34 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col, 
       }
     }
   }
+  // Lanes r and r + 32 share row r of S: the row lane stores columns c <= r,
+  // the inverse lane columns c > r, i.e. lane stores columns lo .. lo + span
+  // (none for inverse lane 31: lo = 32).  Every lane stores every entry, the
+  // ones outside its range into its own entry of col (free by now; distinct
+  // entries, so no bank conflicts): one compare and one address select per
+  // entry, no branches (the range is hidden from the optimiser, which would
+  // otherwise branch on the lane kind per entry).
+  {
+    lds_t* srow = (lds_t*)S + r * pitch;
+    lds_t* sink = colv + lane;
+    int lo = rowlane ? 0 : r + 1, span = rowlane ? r : (r < 31 ? 30 - r : 0);
+    asm volatile("" : "+v"(lo), "+v"(span));
 #pragma unroll
-  for (int c = 0; c < 32; ++c)
-    if (rowlane ? (c <= r) : (c > r)) S[r * pitch + c] = u[c];
-  if (rowlane) dinv[r] = rinv;
+    for (int c = 0; c < 32; ++c) {
+      lds_t* d = ((unsigned)(c - lo) <= (unsigned)span) ? srow + c : sink;
+      asm volatile("" : "+v"(d));
+      *d = u[c];
+    }
+  }
   // ut[r][c] = Linv[c][r] in full (zeros above the diagonal, 1/L[r][r] on
   // it): the inverse lanes' whole columns, the operand of the panel product
   if (!rowlane)
 #pragma unroll
     for (int c = 0; c < 32; ++c) ut[r * 33 + c] = u[c];
+  // dinv from the diagonal of ut (this wavefront's LDS writes above complete
+  // first): tracking 1/L[r][r] through the pair steps by lane selects kept
+  // 32 lane masks alive to the end
+  if (rowlane) dinv[r] = ut[r * 33 + r];
 }
 
 // Inverse of a 32x32 lower-triangular tile (no factorisation), by one
