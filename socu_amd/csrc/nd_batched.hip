@@ -44,7 +44,7 @@ struct BSlot {
   T B[NT * (NT + 1)];
   T C[NT * (NT + 1)];
   T D[NT * (NT + 1)];
-  T col[NT];  // column broadcast buffer
+  T col[2 * NT];  // column broadcast buffer (and the tile factors' store sink)
   T dinv[NT];
 };
 

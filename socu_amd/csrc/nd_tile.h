@@ -441,36 +441,61 @@ __device__ __forceinline__ void tile_tri_inv(T* S, int pitch, T* dinv) {
 
 // Cholesky and inverse of a 16x16 tile by lanes 0-31 of the wavefront (row
 // lanes 0-15, inverse lanes 16-31; lanes 32-63 must not call it), with the
-// recurrence and in/out convention of tile_chol_inv.
+// recurrence and in/out convention of tile_chol_inv, one pivot per step.
+// col: 32 entries of wave-private LDS (the column buffer; all 32 are
+// overwritten at the end).  As in tile_chol_inv, row lanes keep their strict
+// upper entries unmasked (only their own updates touch them and they are not
+// stored), the lane comparisons are made per step on a copy of the row index
+// the compiler cannot see through, and the loads and stores carry no
+// per-entry branches.
 template <typename T>
 __device__ __forceinline__ void tile_chol_inv16(T* S, int pitch, T* dinv, T* col) {
   const int lane = lane_id();
   const int r = lane & 15;
   const bool rowlane = lane < 16;
+  typedef __attribute__((address_space(3))) T lds_t;
   T u[16];
   T rinv = T(1);
 #pragma unroll
-  for (int c = 0; c < 16; ++c) {
-    const T v = S[r * pitch + c];
-    u[c] = rowlane ? ((c <= r) ? v : T(0)) : ((c == r) ? T(1) : T(0));
+  for (int c = 0; c < 16; ++c) u[c] = S[r * pitch + c];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) asm volatile("" : "+v"(u[c]));
+  if (!rowlane) {
+#pragma unroll
+    for (int c = 0; c < 16; ++c) u[c] = (c == r) ? T(1) : T(0);
   }
+  const int rsel = rowlane ? r : 64;  // inverse lanes: past every pivot
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     if (rowlane) col[r] = u[k];
     const T p = bcast_lane(u[k], k);
     const T inv = rsqrt_t(p);
-    const T lk = rowlane ? ((r > k) ? u[k] * inv : T(0)) : u[k] * inv;
+    const T lk = u[k] * inv;
     const T coef = lk * inv;
 #pragma unroll
     for (int q = k + 1; q < 16; ++q) u[q] -= coef * col[q];
-    u[k] = rowlane ? ((r == k) ? p * inv : ((r > k) ? lk : u[k])) : lk;
-    rinv = (r == k) ? inv : rinv;
+    int rk = rsel;
+    asm volatile("" : "+v"(rk));
+    u[k] = (rk == k) ? p * inv : lk;
+    rinv = (rk == k) ? inv : rinv;
+    asm volatile("" : "+v"(rinv));
 #pragma unroll
     for (int q = 0; q < 16; ++q) asm volatile("" : "+v"(u[q]));
   }
+  // row lane r stores columns c <= r, inverse lane r columns c > r
+  // (lo .. lo + span; none for inverse lane 15); the rest go to col[lane]
+  {
+    lds_t* srow = (lds_t*)S + r * pitch;
+    lds_t* sink = (lds_t*)col + lane;
+    int lo = rowlane ? 0 : r + 1, span = rowlane ? r : (r < 15 ? 14 - r : 0);
+    asm volatile("" : "+v"(lo), "+v"(span));
 #pragma unroll
-  for (int c = 0; c < 16; ++c)
-    if (rowlane ? (c <= r) : (c > r)) S[r * pitch + c] = u[c];
+    for (int c = 0; c < 16; ++c) {
+      lds_t* d = ((unsigned)(c - lo) <= (unsigned)span) ? srow + c : sink;
+      asm volatile("" : "+v"(d));
+      *d = u[c];
+    }
+  }
   if (rowlane) dinv[r] = rinv;
 }
 
@@ -531,12 +556,13 @@ __device__ __forceinline__ void tile_chol_inv32b(T* S, int pitch, T* dinv, T* co
 // Cholesky and inverse of G = 64/(2*NT) independent NT x NT tiles per
 // wavefront (NT = 8, 16 or 32): lanes [g*2NT, g*2NT+NT) keep the rows of tile
 // g, lanes [g*2NT+NT, (g+1)*2NT) its inverse columns, with the same recurrence
-// as tile_chol_inv.  Tile g is S0 + g*stride (row-major, pitch `pitch`), its
-// NT-entry column buffer col0 + g*stride and its 1/L[r][r] output
-// dinv0 + g*stride.  The pivot comes through the column buffer, since each
-// group has its own source lane.
+// as tile_chol_inv16.  Tile g is S0 + g*stride (row-major, pitch `pitch`), its
+// 2NT-entry column buffer col0 + g*stride (all 2NT entries are overwritten at
+// the end) and its 1/L[r][r] output dinv0 + g*stride.  The pivot comes
+// through the column buffer, since each group has its own source lane.
 template <typename T, int NT>
 __device__ __forceinline__ void tile_chol_inv_grouped(T* S0, T* col0, T* dinv0, int stride, int pitch) {
+  typedef __attribute__((address_space(3))) T lds_t;
   const int lane = lane_id();
   const int g = lane / (2 * NT);
   const int lr = lane % (2 * NT);
@@ -547,10 +573,14 @@ __device__ __forceinline__ void tile_chol_inv_grouped(T* S0, T* col0, T* dinv0, 
   T u[NT];
   T rinv = T(1);
 #pragma unroll
-  for (int c = 0; c < NT; ++c) {
-    const T v = St[r * pitch + c];
-    u[c] = rowlane ? ((c <= r) ? v : T(0)) : ((c == r) ? T(1) : T(0));
+  for (int c = 0; c < NT; ++c) u[c] = St[r * pitch + c];
+#pragma unroll
+  for (int c = 0; c < NT; ++c) asm volatile("" : "+v"(u[c]));
+  if (!rowlane) {
+#pragma unroll
+    for (int c = 0; c < NT; ++c) u[c] = (c == r) ? T(1) : T(0);
   }
+  const int rsel = rowlane ? r : 64;
 #pragma unroll
   for (int k = 0; k < NT; ++k) {
     if (rowlane) cb[r] = u[k];
@@ -558,19 +588,31 @@ __device__ __forceinline__ void tile_chol_inv_grouped(T* S0, T* col0, T* dinv0, 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const T p = cb[k];
     const T inv = rsqrt_t(p);
-    const T lk = rowlane ? ((r > k) ? u[k] * inv : T(0)) : u[k] * inv;
+    const T lk = u[k] * inv;
     const T coef = lk * inv;
 #pragma unroll
     for (int q = k + 1; q < NT; ++q) u[q] -= coef * cb[q];
-    u[k] = rowlane ? ((r == k) ? p * inv : ((r > k) ? lk : u[k])) : lk;
-    rinv = (r == k) ? inv : rinv;
+    int rk = rsel;
+    asm volatile("" : "+v"(rk));
+    u[k] = (rk == k) ? p * inv : lk;
+    rinv = (rk == k) ? inv : rinv;
+    asm volatile("" : "+v"(rinv));
 #pragma unroll
     for (int q = 0; q < NT; ++q) asm volatile("" : "+v"(u[q]));
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   }
+  {
+    lds_t* srow = (lds_t*)St + r * pitch;
+    lds_t* sink = (lds_t*)cb + lr;
+    int lo = rowlane ? 0 : r + 1, span = rowlane ? r : (r < NT - 1 ? NT - 2 - r : 0);
+    asm volatile("" : "+v"(lo), "+v"(span));
 #pragma unroll
-  for (int c = 0; c < NT; ++c)
-    if (rowlane ? (c <= r) : (c > r)) St[r * pitch + c] = u[c];
+    for (int c = 0; c < NT; ++c) {
+      lds_t* d = ((unsigned)(c - lo) <= (unsigned)span) ? srow + c : sink;
+      asm volatile("" : "+v"(d));
+      *d = u[c];
+    }
+  }
   if (rowlane) dinv0[g * stride + r] = rinv;
 }
 
