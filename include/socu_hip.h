@@ -137,6 +137,24 @@ int socu_cholesky_solve_noalloc(const void* L, const void* E, void* x, const uns
 int socu_cholesky_factor_and_solve(void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
                                    int dtype, void* stream);
 
+/* Out-of-place factor and factor_and_solve.  No counterpart in the
+ * reference (whose factor is in place, block_tridiag_solver.py:864-918);
+ * for callers that must keep their inputs, such as a differentiable solve
+ * (the backward pass needs the factor, the caller's L and E stay as given):
+ * the factor of (L_src, E_src) is written to (L, E), which need not hold
+ * anything on entry, without copying the inputs first (the first level reads
+ * them, every later one reads L and E).  Only tril(L_src) and the coupling
+ * blocks E_src[b, :N-1] are read; the sources are not written.  L / E must
+ * not overlap L_src / E_src.  Masked-out systems are not written.  Results
+ * equal those of socu_cholesky_factor / socu_cholesky_factor_and_solve on
+ * copies of the inputs.  (For n > 128 the inputs are copied, then factored
+ * in place.) */
+int socu_cholesky_factor_from(const void* L_src, const void* E_src, void* L, void* E, const unsigned char* mask, int B,
+                              int N, int n, int dtype, void* stream);
+int socu_cholesky_factor_and_solve_from(const void* L_src, const void* E_src, void* L, void* E, void* x,
+                                        const unsigned char* mask, int B, int N, int n, int m, int dtype,
+                                        void* stream);
+
 /* A plan binds one operation to its arrays, like the reference's launch
  * callables (arrays bound at creation).  With use_graph != 0 the work is
  * captured once into a hipGraph on a private capture stream and replayed by
@@ -226,13 +244,14 @@ int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask
  * bytes indexed like `systems`.  One workgroup per system walks all its
  * levels.  Systems fall in size buckets by max(n, m) (<= 8, <= 16, <= 32);
  * each bucket is one launch, with a tile size of at least its bound and
- * workgroups of 1 or 4 wavefronts, the launches on forked streams.  With
+ * workgroups of 1, 2 or 4 wavefronts, the launches on forked streams.  With
  * `tune` the create call times every admissible (tile size, wavefronts) per
  * bucket on a device copy of the batch (the caller's arrays are only read)
  * and keeps the fastest; results are cached per process and, if the
  * environment variable SOCU_TUNE_CACHE names a file, appended there and
  * reused by later processes.  Without `tune` each bucket uses its own bound
- * and one wavefront.  create copies the system table to the device and
+ * and one wavefront (or the count in the environment variable
+ * SOCU_BATCHED_WAVES: 1, 2 or 4).  create copies the system table to the device and
  * synchronises; execute allocates nothing and synchronises nothing.
  * socu_batched_config returns per bucket the system count, tile size and
  * wavefronts per workgroup. */

@@ -61,6 +61,8 @@ SIGNATURES = {
     "socu_cholesky_solve_with_inverses": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "socu_cholesky_solve_noalloc": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "socu_cholesky_factor_and_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "socu_cholesky_factor_from": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp]),
+    "socu_cholesky_factor_and_solve_from": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "socu_plan_create": (_i, [ctypes.POINTER(_vp), _i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i]),
     "socu_plan_execute": (_i, [_vp, _vp]),
     "socu_plan_destroy": (_i, [_vp]),

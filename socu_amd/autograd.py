@@ -54,16 +54,17 @@ class BlockTridiagSolve(torch.autograd.Function):
         L4, E4, b4, B, N, n, m = _batched(L.detach(), E.detach(), b.detach())
         code = _code(L)
         lib = _lib.load()
-        Lw = L4.contiguous().clone()
-        # the fill-in region E[N-1:] is written by the factor before it is
-        # read, so only the coupling blocks are copied
-        Ew = torch.empty_like(E4, memory_format=torch.contiguous_format)
-        if N > 1:
-            Ew[:, : N - 1] = E4[:, : N - 1]
+        # the factor goes to fresh buffers, read straight from the inputs
+        # (out of place: no copies of L and E), which stay as given
+        Ls = L4.contiguous()
+        Es = E4.contiguous()
+        Lw = torch.empty_like(Ls)
+        Ew = torch.empty_like(Es)
         x = b4.contiguous().clone()
         with torch.cuda.device(L.device):
-            _lib.check(lib.socu_cholesky_factor_and_solve(Lw.data_ptr(), Ew.data_ptr(), x.data_ptr(), None, B, N, n, m,
-                                                          code, _stream(L.device)), "block_tridiag_solve")
+            _lib.check(lib.socu_cholesky_factor_and_solve_from(Ls.data_ptr(), Es.data_ptr(), Lw.data_ptr(),
+                                                               Ew.data_ptr(), x.data_ptr(), None, B, N, n, m, code,
+                                                               _stream(L.device)), "block_tridiag_solve")
         ctx.save_for_backward(Lw, Ew, x)
         ctx.dims = (B, N, n, m, code, L.shape, E.shape, shape_b)
         return x.reshape(shape_b)

@@ -104,7 +104,7 @@ def create_batched_cholesky_launch(L: Sequence[torch.Tensor], E: Sequence[torch.
     ``mask`` is a bool tensor with one entry per system; a False entry leaves
     that system untouched, as in the reference's mask (README.md:214-235).
     ``tune`` times every admissible tile size (8, 16, 32, at least the
-    bucket's bound) and workgroup size (1 or 4 wavefronts) for each size
+    bucket's bound) and workgroup size (1, 2 or 4 wavefronts) for each size
     bucket on a device copy of the batch when the launch is created and keeps
     the fastest (cached per process, and in the file named by
     SOCU_TUNE_CACHE).

@@ -59,10 +59,12 @@ int check_dims(const void* L, const void* E, const void* x, bool need_x, int B, 
 }
 
 template <typename T>
-int run_factor(T* L, T* E, const uint8_t* mask, int B, int N, int n, hipStream_t st) {
+int run_factor(T* L, T* E, const uint8_t* mask, int B, int N, int n, hipStream_t st, const T* Lsrc = nullptr,
+               const T* Esrc = nullptr) {
   const int Lv = socu::recursive_iterations(N);
   for (int l = 0; l < Lv; ++l) {
-    hipError_t e = socu::factor_level_blocked<T>(L, E, mask, B, N, n, l, false, st);
+    hipError_t e = socu::factor_level_blocked<T>(L, E, mask, B, N, n, l, false, st, nullptr, nullptr, nullptr,
+                                                 l == 0 ? Lsrc : nullptr, l == 0 ? Esrc : nullptr);
     if (e != hipSuccess) return hip_fail(e, "factor level launch");
   }
   return SOCU_OK;
@@ -114,11 +116,26 @@ int run_sweeps(const T* L, const T* E, T* x, const uint8_t* mask, int B, int N, 
 
 template <typename T>
 int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N, int n, int m, const SweepWs& sw,
-           const socu::SweepOverlap* ov, void* inv, hipStream_t st) {
+           const socu::SweepOverlap* ov, void* inv, hipStream_t st, const void* Lsrc = nullptr,
+           const void* Esrc = nullptr) {
   T* Lt = static_cast<T*>(L);
   T* Et = static_cast<T*>(E);
   T* xt = static_cast<T*>(x);
+  const T* Ls = static_cast<const T*>(Lsrc);
+  const T* Es = static_cast<const T*>(Esrc);
   int rc = SOCU_OK;
+  if (Ls != nullptr && n > socu::kMaxN) {
+    // the tiled kernels of n > 128 factor in place only: copy, then factor
+    const size_t nn = (size_t)n * n, es = sizeof(T);
+    hipError_t e = hipMemcpyAsync(Lt, Ls, (size_t)B * N * nn * es, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess && N > 1) {
+      const size_t pitch = (size_t)socu::off_diag_storage_len(N) * nn * es;
+      e = hipMemcpy2DAsync(Et, pitch, Es, pitch, (size_t)(N - 1) * nn * es, B, hipMemcpyDeviceToDevice, st);
+    }
+    if (e != hipSuccess) return hip_fail(e, "input copy of the out-of-place factor");
+    Ls = nullptr;
+    Es = nullptr;
+  }
   if ((op == SOCU_OP_FORWARD || op == SOCU_OP_SOLVE || op == SOCU_OP_BACKWARD) && inv != nullptr) {
     // The sweeps multiply by the inverses of the diagonal factor blocks;
     // they are formed here from tril(L), so the factor may come from
@@ -133,7 +150,7 @@ int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N,
   }
   switch (op) {
     case SOCU_OP_FACTOR:
-      return run_factor<T>(Lt, Et, mask, B, N, n, st);
+      return run_factor<T>(Lt, Et, mask, B, N, n, st, Ls, Es);
     case SOCU_OP_FORWARD:
       return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, true, false, sw, st);
     case SOCU_OP_BACKWARD:
@@ -142,11 +159,12 @@ int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N,
       return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, true, true, sw, st);
     case SOCU_OP_FACTOR_AND_SOLVE:
       if (ov != nullptr) {  // forward sweep alongside the factor, then the backward sweep
-        const hipError_t e = socu::factor_with_forward<T>(Lt, Et, xt, mask, mask, B, N, n, m, false, *ov, st);
+        const hipError_t e =
+            socu::factor_with_forward<T>(Lt, Et, xt, mask, mask, B, N, n, m, false, *ov, st, nullptr, Ls, Es);
         if (e != hipSuccess) return hip_fail(e, "factor_and_solve launch");
         return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, false, true, sw, st);
       }
-      rc = run_factor<T>(Lt, Et, mask, B, N, n, st);
+      rc = run_factor<T>(Lt, Et, mask, B, N, n, st, Ls, Es);
       if (rc) return rc;
       return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, true, true, sw, st);
     default:
@@ -179,7 +197,7 @@ bool op_needs_inverse(int op) { return op == SOCU_OP_FORWARD || op == SOCU_OP_SO
 
 int dispatch(int op, void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m, int dtype,
              void* stream, void* inv, const SweepWs& sw = SweepWs(), const socu::SweepOverlap* ov = nullptr,
-             bool stored_inverse = false) {
+             bool stored_inverse = false, const void* Lsrc = nullptr, const void* Esrc = nullptr) {
   const bool need_x = op != SOCU_OP_FACTOR;
   int rc = check_dims(L, E, x, need_x, B, N, n, m, dtype);
   if (rc) return rc;
@@ -189,8 +207,8 @@ int dispatch(int op, void* L, void* E, void* x, const unsigned char* mask, int B
   rc = ensure_init();
   if (rc) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == SOCU_DTYPE_F64) return run_op<double>(op, L, E, x, mask, B, N, n, m, sw, ov, inv, st);
-  return run_op<float>(op, L, E, x, mask, B, N, n, m, sw, ov, inv, st);
+  if (dtype == SOCU_DTYPE_F64) return run_op<double>(op, L, E, x, mask, B, N, n, m, sw, ov, inv, st, Lsrc, Esrc);
+  return run_op<float>(op, L, E, x, mask, B, N, n, m, sw, ov, inv, st, Lsrc, Esrc);
 }
 
 // Per-thread launch timer used by socu_profile_op.
@@ -441,6 +459,43 @@ int socu_cholesky_solve_noalloc(const void* L, const void* E, void* x, const uns
 int socu_cholesky_factor_and_solve(void* L, void* E, void* x, const unsigned char* mask, int B, int N, int n, int m,
                                    int dtype, void* stream) {
   return dispatch(SOCU_OP_FACTOR_AND_SOLVE, L, E, x, mask, B, N, n, m, dtype, stream, nullptr);
+}
+
+// the sources of an out-of-place factor: given, and disjoint from L / E
+static int check_sources(const void* Ls, const void* Es, const void* L, const void* E, int B, int N, int n,
+                         int dtype) {
+  if (B == 0) return SOCU_OK;
+  if (Ls == nullptr || (Es == nullptr && N > 1)) return fail(SOCU_EINVAL, "L_src or E_src is NULL");
+  const size_t es = dtype == SOCU_DTYPE_F64 ? sizeof(double) : sizeof(float);
+  const size_t nn = (size_t)n * n * es;
+  auto overlap = [](const void* a, size_t na, const void* b, size_t nb) {
+    const char* p = static_cast<const char*>(a);
+    const char* q = static_cast<const char*>(b);
+    return p < q + nb && q < p + na;
+  };
+  const size_t nL = (size_t)B * N * nn, nE = (size_t)B * socu::off_diag_storage_len(N) * nn;
+  if (overlap(Ls, nL, L, nL) || (N > 1 && (overlap(Es, nE, E, nE) || overlap(Ls, nL, E, nE) || overlap(Es, nE, L, nL))))
+    return fail(SOCU_EINVAL, "L_src / E_src overlap L / E");
+  return SOCU_OK;
+}
+
+int socu_cholesky_factor_from(const void* L_src, const void* E_src, void* L, void* E, const unsigned char* mask, int B,
+                              int N, int n, int dtype, void* stream) {
+  int rc = check_dims(L, E, nullptr, false, B, N, n, 1, dtype);
+  if (rc == SOCU_OK) rc = check_sources(L_src, E_src, L, E, B, N, n, dtype);
+  if (rc) return rc;
+  return dispatch(SOCU_OP_FACTOR, L, E, nullptr, mask, B, N, n, 1, dtype, stream, nullptr, SweepWs(), nullptr, false,
+                  L_src, E_src);
+}
+
+int socu_cholesky_factor_and_solve_from(const void* L_src, const void* E_src, void* L, void* E, void* x,
+                                        const unsigned char* mask, int B, int N, int n, int m, int dtype,
+                                        void* stream) {
+  int rc = check_dims(L, E, x, true, B, N, n, m, dtype);
+  if (rc == SOCU_OK) rc = check_sources(L_src, E_src, L, E, B, N, n, dtype);
+  if (rc) return rc;
+  return dispatch(SOCU_OP_FACTOR_AND_SOLVE, L, E, x, mask, B, N, n, m, dtype, stream, nullptr, SweepWs(), nullptr,
+                  false, L_src, E_src);
 }
 
 int socu_adjoint_scatter(const void* bbar, const void* x, void* dL, void* dE, const unsigned char* mask, int B, int N,
@@ -767,7 +822,7 @@ void tune_cache_load() {
   char key[256];
   int c = 0;
   while (fscanf(f, "%255s %d", key, &c) == 2)
-    if (c >= 0 && (c & 7) < kClasses && ((c >> 3) == 1 || (c >> 3) == 4)) g_tune[key] = c;
+    if (c >= 0 && (c & 7) < kClasses && ((c >> 3) == 1 || (c >> 3) == 2 || (c >> 3) == 4)) g_tune[key] = c;
   fclose(f);
 }
 
@@ -881,7 +936,7 @@ int batched_enqueue(socu_batched* h, const unsigned char* mask, hipStream_t st) 
 }
 
 // For each size bucket, times every admissible (tile class >= the bucket's
-// own, wavefronts 1 or 4) on a device copy of the batch and keeps the
+// own, wavefronts 1, 2 or 4) on a device copy of the batch and keeps the
 // fastest.  The caller's arrays are only read.  Needs batched_layout first.
 int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
@@ -941,7 +996,7 @@ int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) 
     float best = 1e30f;
     int best_c = b, best_w = 1;
     for (int c = b; c < kClasses && e == hipSuccess; ++c) {
-      for (int w : {1, 4}) {
+      for (int w : {1, 2, 4}) {
         for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
           e = socu::batched_copy(h->dprobs, dcopy, h->count, h->op, h->dtype, ts);
           if (e == hipSuccess) e = hipEventRecord(e0, ts);
@@ -1041,6 +1096,14 @@ int socu_batched_create(socu_batched** out, int op, const socu_batch_system* sys
     return hip_fail(e, "batched plan set-up");
   }
   rc = batched_layout(h);
+  if (rc == SOCU_OK && !tune) {
+    // SOCU_BATCHED_WAVES=1|2|4 sets the wavefronts per workgroup of an
+    // untuned plan (every variant of the kernels can then be run on purpose)
+    const char* w = getenv("SOCU_BATCHED_WAVES");
+    const int wv = w ? atoi(w) : 1;
+    if (wv == 1 || wv == 2 || wv == 4)
+      for (int b = 0; b < kClasses; ++b) h->waves[b] = wv;
+  }
   if (rc == SOCU_OK && tune) rc = batched_tune(h, probs);
   if (rc) {
     batched_release(h);

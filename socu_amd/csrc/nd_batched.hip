@@ -24,7 +24,7 @@
 // x_i <- L_i^-T (x_i - R_i^T x_{i+s} - Lc_i x_{i-s}).
 //
 // Tiles are NT x (NT+1) in LDS, NT in {8, 16, 32}, zero padded beyond n (the
-// diagonal block identity padded).  A workgroup has W = 1 or 4 wavefronts and
+// diagonal block identity padded).  A workgroup has W = 1, 2 or 4 wavefronts and
 // runs one or more tasks at once (BShape); block products run on MFMA
 // (16x16 tiles, NT >= 16) or, for NT = 8, as 2x2 register blocks per lane.
 // Small workgroups (W = 1) let several systems share a compute unit, one per
@@ -44,7 +44,7 @@ struct BSlot {
   T B[NT * (NT + 1)];
   T C[NT * (NT + 1)];
   T D[NT * (NT + 1)];
-  T col[2 * NT];  // column broadcast buffer (and the tile factors' store sink)
+  T col[2 * NT];  // column buffer and store sink of the tile factors
   T dinv[NT];
 };
 
@@ -94,12 +94,6 @@ __device__ __forceinline__ void tcommit(T* t, T* v, const T* src, int n, int q, 
   }
 }
 
-template <typename T, int NT, int W>
-__device__ __forceinline__ void tload(T* t, const T* __restrict__ src, int n, int q, bool ident) {
-  T v[BShape<NT, W>::EPL];
-  tfetch<T, NT, W>(v, src, n, q);
-  tcommit<T, NT, W>(t, v, src, n, q, ident);
-}
 
 // Position (i, j) of the e-th output element tprod hands this thread, in the
 // order of its out() calls.
@@ -128,12 +122,31 @@ __device__ __forceinline__ void pzero(T* v) {
 }
 
 // Vector block (n x m, row-major) into a tile of pitch yp; rows >= n zero.
+// The loads are unconditional (clamped rows) and pinned before the select:
+// a select on each load lets the compiler turn it into a branch with a wait.
 template <typename T, int NT, int W>
 __device__ __forceinline__ void yload(T* t, int yp, const T* __restrict__ src, int n, int m, int q) {
   constexpr int LPT = BShape<NT, W>::LPT;
   for (int idx = q; idx < NT * m; idx += LPT) {
     const int r = idx / m, c = idx - r * m;
-    t[r * yp + c] = (src != nullptr && r < n) ? src[(size_t)r * m + c] : T(0);
+    T v = T(0);
+    if (src != nullptr) {
+      v = src[(size_t)min(r, n - 1) * m + c];
+      pin_reg(v);
+    }
+    t[r * yp + c] = (r < n) ? v : T(0);
+  }
+}
+
+// Tile (NT x NT, pitch TP) to an n x n block of HBM, rows and columns below
+// n only (no division by the runtime n).
+template <typename T, int NT, int W>
+__device__ __forceinline__ void tstore(T* __restrict__ dst, const T* t, int n, int q) {
+  constexpr int TP = BShape<NT, W>::TP, LPT = BShape<NT, W>::LPT;
+#pragma unroll
+  for (int idx = q; idx < NT * NT; idx += LPT) {
+    const int r = idx / NT, c = idx % NT;
+    if (r < n && c < n) dst[(size_t)r * n + c] = t[r * TP + c];
   }
 }
 
@@ -236,15 +249,17 @@ __device__ __forceinline__ void vprod(int n, int m, int kend, FA fa, FX fx, bool
 }
 
 // Cholesky + inverse of every slot's A tile (lower: L, strict upper: L^-T,
-// dinv: 1/L[r][r]).
+// dinv: 1/L[r][r]); `node` is this thread's task kind (a deferred target task
+// has no factor).  (NT = 32 keeps the 16x16-halves factor: the 32-wide pivot
+// pairs of tile_chol_inv need registers this kernel does not have.)
 template <typename T, int NT, int W>
-__device__ __forceinline__ void slots_chol(BSlot<T, NT>* slots) {
+__device__ __forceinline__ void slots_chol(BSlot<T, NT>* slots, bool node) {
   constexpr int TP = BShape<NT, W>::TP;
-  if constexpr (NT == 32) {  // one task per workgroup; its first wavefront
-    if (wave_id() == 0) tile_chol_inv32b<T>(slots[0].A, TP, slots[0].dinv, slots[0].col);
-  } else if constexpr (NT == 16) {  // one task per wavefront
+  if constexpr (NT == 32) {  // one task per workgroup (node is uniform); its first wavefront
+    if (node && wave_id() == 0) tile_chol_inv32b<T>(slots[0].A, TP, slots[0].dinv, slots[0].col);
+  } else if constexpr (NT == 16) {  // one task per wavefront (node is wavefront-uniform)
     BSlot<T, NT>& S = slots[wave_id()];
-    if (lane_id() < 32) tile_chol_inv16<T>(S.A, TP, S.dinv, S.col);
+    if (node && lane_id() < 32) tile_chol_inv16<T>(S.A, TP, S.dinv, S.col);
   } else {  // four tasks per wavefront
     constexpr int stride = (int)(sizeof(BSlot<T, NT>) / sizeof(T));
     static_assert(sizeof(BSlot<T, NT>) % sizeof(T) == 0, "slot stride");
@@ -277,7 +292,11 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
   T* Dt = S.D;
   T* dv = S.dinv;
   // Linv[i][k] of a factored tile: its strict upper triangle holds Linv^T
-  auto linv = [&](int i, int k) -> T { return (k < i) ? At[k * TP + i] : ((k == i) ? dv[i] : T(0)); };
+  // (both reads unconditional: a select on a read becomes a branch)
+  auto linv = [&](int i, int k) -> T {
+    const T u = At[k * TP + i], d = dv[i];
+    return (k < i) ? u : ((k == i) ? d : T(0));
+  };
 
   if (fac || fwd) {
     int offp = 0, off = 0;  // E segment offsets of levels l-1 and l
@@ -310,30 +329,53 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
         const T* Rr = (up && has_r) ? E + (size_t)(offp + ((k + h) >> sh)) * nn : nullptr;     // (k+s, k+h)
         T* yk = (fwd && task) ? x + (size_t)k * nm : nullptr;
 
-        // ---- P1: loads (tiles in pairs, each pair's loads in flight together) ----
+        // ---- P1: loads, every tile's (and the vectors') in flight at once ----
         constexpr int EPL = BShape<NT, W>::EPL;  // tile / product elements per lane
         {
           // level 0 (node): B <- R_k, D <- Lc_k (original couplings);
           // level > 0: B <- R_{k-h}, C <- Lc_{k+h}, D <- Lc_{k-h}
           const T* srcB = up ? Rl : ((node && fac) ? Rk : nullptr);
           const T* srcD = up ? (fac ? Lcl : nullptr) : ((node && fac) ? Lck : nullptr);
-          if constexpr (EPL <= 4) {  // pairs of tiles in flight
-            T va[EPL], vb[EPL];
+          auto ydv = [&]() {  // vectors, and 1/L[q][q] for solves of a stored factor
+            if (fwd) {
+              yload<T, NT, W>(Y0, yp, yk, n, m, q);
+              if (up) {
+                yload<T, NT, W>(Y1, yp, x + (size_t)(k - h) * nm, n, m, q);
+                yload<T, NT, W>(Y2, yp, right ? x + (size_t)(k + h) * nm : nullptr, n, m, q);
+              }
+            }
+            if (!fac && node && q < NT) {
+              T d = Lk[(size_t)min(q, n - 1) * (n + 1)];
+              pin_reg(d);
+              dv[q] = (q < n) ? T(1) / d : T(1);
+            }
+          };
+          const T* srcC = up ? Lcr : nullptr;
+          if (!(fac && (up || node))) srcD = nullptr;
+          if constexpr (EPL * sizeof(T) <= 64) {  // every load in flight at once
+            T va[EPL], vb[EPL], vc[EPL], vd[EPL];
             tfetch<T, NT, W>(va, Lk, n, q);
             tfetch<T, NT, W>(vb, srcB, n, q);
+            tfetch<T, NT, W>(vc, srcC, n, q);
+            tfetch<T, NT, W>(vd, srcD, n, q);
+            ydv();
             tcommit<T, NT, W>(At, va, Lk, n, q, true);
             if (up || (node && fac)) tcommit<T, NT, W>(Bt, vb, srcB, n, q, false);
-            if (up || (node && fac)) {
-              tfetch<T, NT, W>(va, up ? Lcr : nullptr, n, q);
-              tfetch<T, NT, W>(vb, srcD, n, q);
-              if (up) tcommit<T, NT, W>(Ct, va, Lcr, n, q, false);
-              if (fac) tcommit<T, NT, W>(Dt, vb, srcD, n, q, false);
-            }
-          } else {  // 16 elements per lane: one tile at a time (registers)
-            tload<T, NT, W>(At, Lk, n, q, true);
-            if (up || (node && fac)) tload<T, NT, W>(Bt, srcB, n, q, false);
-            if (up) tload<T, NT, W>(Ct, Lcr, n, q, false);
-            if (fac && (up || node)) tload<T, NT, W>(Dt, srcD, n, q, false);
+            if (up) tcommit<T, NT, W>(Ct, vc, srcC, n, q, false);
+            if (fac && (up || node)) tcommit<T, NT, W>(Dt, vd, srcD, n, q, false);
+          } else {
+            // FP64 tiles of 16 elements per lane (NT = 32, one wavefront): one
+            // tile at a time; two in flight spill registers in this kernel
+            auto tl = [&](T* t, const T* sp, bool ident) {
+              T v[EPL];
+              tfetch<T, NT, W>(v, sp, n, q);
+              tcommit<T, NT, W>(t, v, sp, n, q, ident);
+            };
+            tl(At, Lk, true);
+            if (up || (node && fac)) tl(Bt, srcB, false);
+            if (up) tl(Ct, srcC, false);
+            if (fac && (up || node)) tl(Dt, srcD, false);
+            ydv();
           }
         }
         // level > 0: node k's couplings are fill-in blocks that receive their
@@ -341,14 +383,6 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
         // the fill products start from zero and the fill-in region's contents
         // on entry are never read
         T old[EPL];
-        if (fwd) {
-          yload<T, NT, W>(Y0, yp, yk, n, m, q);
-          if (up) {
-            yload<T, NT, W>(Y1, yp, x + (size_t)(k - h) * nm, n, m, q);
-            yload<T, NT, W>(Y2, yp, right ? x + (size_t)(k + h) * nm : nullptr, n, m, q);
-          }
-        }
-        if (!fac && node && q < NT) dv[q] = (q < n) ? T(1) / Lk[(size_t)q * n + q] : T(1);
         __syncthreads();
 
         // ---- P2: updates from level l-1 ----
@@ -391,8 +425,7 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
           if (has_r) pzero<T, NT, W>(old);
         }
         if (defer) {
-          if (fac)
-            for (int idx = q; idx < n * n; idx += LPT) Lk[idx] = At[(idx / n) * TP + idx % n];
+          if (fac) tstore<T, NT, W>(Lk, At, n, q);
           if (fwd)
             for (int idx = q; idx < n * m; idx += LPT) yk[idx] = Y0[(idx / m) * yp + idx % m];
         }
@@ -421,14 +454,14 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
 
         // ---- P5: factor and invert the diagonal blocks ----
         if (fac) {
-          slots_chol<T, NT, W>(slots);
+          slots_chol<T, NT, W>(slots, node);
           __syncthreads();
         }
 
         // ---- P6: node outputs ----
         if (node) {
           if (fac) {
-            for (int idx = q; idx < n * n; idx += LPT) Lk[idx] = At[(idx / n) * TP + idx % n];
+            tstore<T, NT, W>(Lk, At, n, q);
             if (has_lc)  // Lc_k <- Linv_k Lc_k
               tprod1<T, NT, W>(
                   linv, [&](int kk, int j) { return Dt[kk * TP + j]; },
@@ -468,22 +501,30 @@ __device__ __forceinline__ void batched_system(const BatchProblem& P, int op, un
         {
           const T* Ri = right ? E + (size_t)(off + i / s) * nn : nullptr;       // R_i
           const T* Lci = left ? E + (size_t)(off + (i - s) / s) * nn : nullptr;  // Lc_i
-          if constexpr (BShape<NT, W>::EPL <= 4) {
-            T va[BShape<NT, W>::EPL], vb[BShape<NT, W>::EPL];
-            tfetch<T, NT, W>(va, Li, n, q);
-            tfetch<T, NT, W>(vb, Ri, n, q);
+          constexpr bool kAll = BShape<NT, W>::EPL * sizeof(T) <= 64;
+          T va[BShape<NT, W>::EPL], vb[BShape<NT, W>::EPL], vc[BShape<NT, W>::EPL];
+          tfetch<T, NT, W>(va, Li, n, q);
+          tfetch<T, NT, W>(vb, Ri, n, q);
+          if constexpr (!kAll) {
             tcommit<T, NT, W>(At, va, Li, n, q, true);
             tcommit<T, NT, W>(Bt, vb, Ri, n, q, false);
-          } else {
-            tload<T, NT, W>(At, Li, n, q, true);
-            tload<T, NT, W>(Bt, Ri, n, q, false);
+            asm volatile("" ::: "memory");
           }
-          tload<T, NT, W>(Ct, Lci, n, q, false);
+          tfetch<T, NT, W>(vc, Lci, n, q);
+          yload<T, NT, W>(Y0, yp, xi, n, m, q);
+          yload<T, NT, W>(Y1, yp, right ? x + (size_t)(i + s) * nm : nullptr, n, m, q);
+          yload<T, NT, W>(Y2, yp, left ? x + (size_t)(i - s) * nm : nullptr, n, m, q);
+          if (node && q < NT) {
+            T d = Li[(size_t)min(q, n - 1) * (n + 1)];
+            pin_reg(d);
+            dv[q] = (q < n) ? T(1) / d : T(1);
+          }
+          if constexpr (kAll) {
+            tcommit<T, NT, W>(At, va, Li, n, q, true);
+            tcommit<T, NT, W>(Bt, vb, Ri, n, q, false);
+          }
+          tcommit<T, NT, W>(Ct, vc, Lci, n, q, false);
         }
-        yload<T, NT, W>(Y0, yp, xi, n, m, q);
-        yload<T, NT, W>(Y1, yp, right ? x + (size_t)(i + s) * nm : nullptr, n, m, q);
-        yload<T, NT, W>(Y2, yp, left ? x + (size_t)(i - s) * nm : nullptr, n, m, q);
-        if (node && q < NT) dv[q] = (q < n) ? T(1) / Li[(size_t)q * n + q] : T(1);
         __syncthreads();
         // x_i - R_i^T x_{i+s} - Lc_i x_{i-s} (in place: element (r, j) of Y0 is
         // read and written by one lane only)
@@ -531,6 +572,7 @@ hipError_t set_attr() {
 template <typename T, int NT>
 hipError_t set_attrs() {
   hipError_t e = set_attr<T, NT, 1>();
+  if (e == hipSuccess) e = set_attr<T, NT, 2>();
   return e == hipSuccess ? set_attr<T, NT, 4>() : e;
 }
 
@@ -541,6 +583,9 @@ void launch_nt(const BatchProblem* dprobs, const int* dorder, int count, int mma
   if (waves == 1) {
     const size_t lds = lds_bytes<T, NT, 1>(yp);
     hipLaunchKernelGGL((k_batched<T, NT, 1>), dim3(count), dim3(64), lds, st, dprobs, dorder, mask, op, yp);
+  } else if (waves == 2) {
+    const size_t lds = lds_bytes<T, NT, 2>(yp);
+    hipLaunchKernelGGL((k_batched<T, NT, 2>), dim3(count), dim3(128), lds, st, dprobs, dorder, mask, op, yp);
   } else {
     const size_t lds = lds_bytes<T, NT, 4>(yp);
     hipLaunchKernelGGL((k_batched<T, NT, 4>), dim3(count), dim3(256), lds, st, dprobs, dorder, mask, op, yp);
@@ -578,7 +623,8 @@ int batched_class(int n) { return n <= 8 ? 0 : (n <= 16 ? 1 : 2); }
 hipError_t batched_launch_class(const BatchProblem* dprobs, const int* dorder, int cls, int count, int mmax, int waves,
                                 const uint8_t* mask, int op, int dtype, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  if (cls < 0 || cls > 2 || mmax < 1 || mmax > (8 << cls) || (waves != 1 && waves != 4)) return hipErrorInvalidValue;
+  if (cls < 0 || cls > 2 || mmax < 1 || mmax > (8 << cls) || (waves != 1 && waves != 2 && waves != 4))
+    return hipErrorInvalidValue;
   if (dtype == SOCU_DTYPE_F64) {
     if (cls == 0) launch_nt<double, 8>(dprobs, dorder, count, mmax, waves, mask, op, st);
     else if (cls == 1) launch_nt<double, 16>(dprobs, dorder, count, mmax, waves, mask, op, st);

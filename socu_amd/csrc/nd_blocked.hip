@@ -46,7 +46,7 @@ constexpr int potrf_threads(int nb) { return nb == 1 ? 64 : kThreads; }
 
 template <typename TS, int NB, int NF, typename T = TS>
 __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, const uint8_t* __restrict__ mask, int N,
-                                                       int n_, int s) {
+                                                       int n_, int s, const TS* __restrict__ Lsrc) {
   const int n = NF ? NF : n_;
   constexpr int NP = 32 * NB;
   constexpr int P = NP + 1;
@@ -56,6 +56,9 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
   if (masked_out(mask, b)) return;
   const int i = s - 1 + 2 * s * blockIdx.x;
   TS* A = L + ((size_t)b * N + i) * (size_t)n * n;
+  // out-of-place factor: the block is read from Lsrc (same layout), the
+  // factor written to L
+  const TS* Ain = (Lsrc != nullptr) ? Lsrc + ((size_t)b * N + i) * (size_t)n * n : A;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* S = reinterpret_cast<T*>(smem_raw);
   T* dinv = S + NP * P;  // 1/L[r][r] for every row
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
     if (wave == 0) {
       TS v[16];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) v[q] = A[(2 * q + lr) * NF + lc];
+      for (int q = 0; q < 16; ++q) v[q] = Ain[(2 * q + lr) * NF + lc];
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int r = 2 * q + lr;
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
         while (tj > ti) { tj -= ti + 1; ++ti; }
         if (ti < NB) {
 #pragma unroll
-          for (int q = 0; q < 16; ++q) v[k][q] = A[(32 * ti + 2 * q + lr) * NF + 32 * tj + lc];
+          for (int q = 0; q < 16; ++q) v[k][q] = Ain[(32 * ti + 2 * q + lr) * NF + 32 * tj + lc];
         }
       }
 #pragma unroll
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
       const int idx = tid + e * TH;
-      v[e] = (T)A[min(idx, nn - 1)];
+      v[e] = (T)Ain[min(idx, nn - 1)];
     }
     for (int idx = tid; idx < NP * NP; idx += TH) {
       const int r = idx / NP, c = idx - r * NP;
@@ -388,7 +391,7 @@ constexpr int trsm_blk_threads(int nb) { return nb == 4 ? kTrsmThreads : kThread
 template <typename T, int NB, int NF>
 __global__ __launch_bounds__(kTrsmThreads) void k_trsm_blk(const T* __restrict__ L, T* __restrict__ E,
                                                       const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
-                                                      int off) {
+                                                      int off, const T* __restrict__ Esrc) {
   const int n = NF ? NF : n_;
   constexpr int NP = 32 * NB;
   constexpr int TP = 33;  // tile pitch
@@ -405,6 +408,7 @@ __global__ __launch_bounds__(kTrsmThreads) void k_trsm_blk(const T* __restrict__
 
   const T* A = L + ((size_t)b * N + i) * (size_t)n * n;
   T* X = E + ((size_t)b * nE + off + (isR ? i / s : (i - s) / s)) * (size_t)n * n;
+  const T* Xin = (Esrc != nullptr) ? Esrc + (X - E) : X;  // out-of-place: R / Lc read from Esrc
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* Ut = reinterpret_cast<T*>(smem_raw);  // NT upper tiles (kb <= q) of U, 32 x TP each
   T* Xs = Ut + NT * 32 * TP;                // R task: 32 x (NP+1); Lc task: NP x TP
@@ -441,7 +445,7 @@ __global__ __launch_bounds__(kTrsmThreads) void k_trsm_blk(const T* __restrict__
     const int kc = idx >> 5, cc = idx & 31;       // Lc[kc][c0+cc]
     const size_t ar = (size_t)(c0 + min(rr, w - 1)) * n + min(kr, n - 1);
     const size_t ac = (size_t)min(kc, n - 1) * n + c0 + min(cc, w - 1);
-    zv[e] = X[isR ? ar : ac];
+    zv[e] = Xin[isR ? ar : ac];
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t)
@@ -530,7 +534,7 @@ __global__ __launch_bounds__(kTrsmThreads) void k_trsm_blk(const T* __restrict__
 template <typename T, int NB, int NF>
 __global__ __launch_bounds__(kTrsmThreads) void k_trsm_node(const T* __restrict__ L, T* __restrict__ E,
                                                        const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
-                                                       int off, int cpb) {
+                                                       int off, int cpb, const T* __restrict__ Esrc) {
   const int n = NF ? NF : n_;
   constexpr int NP = 32 * NB;
   constexpr int TP = 33;
@@ -565,7 +569,9 @@ __global__ __launch_bounds__(kTrsmThreads) void k_trsm_node(const T* __restrict_
     w = min(32, n - c0);
     X = isR ? XR : XC;
   };
-  auto zload = [&](T* zv, bool isR, int c0, int w, const T* X) {
+  // (out-of-place: R and Lc are read from Esrc, same layout)
+  auto zload = [&](T* zv, bool isR, int c0, int w, const T* Xd) {
+    const T* X = (Esrc != nullptr) ? Esrc + (Xd - E) : Xd;
 #pragma unroll
     for (int e = 0; e < PZ; ++e) {
       const int idx = tid + e * kTrsmThreads;
@@ -787,7 +793,8 @@ __host__ __device__ constexpr int schur_blk_per_xcd(int nT, int nF, int ntile) {
 template <typename T, int NF, bool BF = false>
 __global__ __launch_bounds__(kThreads) void k_schur_blk(T* __restrict__ L, T* __restrict__ E,
                                                        const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
-                                                       int off, int next_off, int nT, int nF, int ntile) {
+                                                       int off, int next_off, int nT, int nF, int ntile,
+                                                       const T* __restrict__ Lsrc) {
   const int n = NF ? NF : n_;
   const int b = blockIdx.y;
   if (masked_out(mask, b)) return;
@@ -845,9 +852,11 @@ __global__ __launch_bounds__(kThreads) void k_schur_blk(T* __restrict__ L, T* __
   if (!active) return;
   const int rb = tr * kSTile + 32 * wi, cb = tc * kSTile + 32 * wj;
   if (target) {
+    // (out-of-place: the target is read from Lsrc, same layout)
+    const T* Cin = (Lsrc != nullptr) ? Lsrc + (C - L) : C;
     acc_foreach<T>(acc, [&](int r, int c, T v) {
       const int rr = rb + r, cc = cb + c;
-      if (rr < n && cc < n) C[(size_t)rr * n + cc] -= v;
+      if (rr < n && cc < n) C[(size_t)rr * n + cc] = Cin[(size_t)rr * n + cc] - v;
     });
   } else {
     // the fill-in block receives its only update here: F = -R Lc, whatever
@@ -915,7 +924,7 @@ __device__ __forceinline__ void panel_store(T* P, const T* v, bool cols, int i0,
 template <typename T, int NB, int NF, bool BF = false>
 __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* __restrict__ E,
                                                          const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
-                                                         int off, int next_off, int nT) {
+                                                         int off, int next_off, int nT, const T* __restrict__ Lsrc) {
   const int n = NF ? NF : n_;
   constexpr int NP = 32 * NB, PP = NP + 1;
   constexpr int PE = 32 * NP / kThreads;
@@ -979,10 +988,12 @@ __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* 
   // F = -R Lc, so its entry contents are not read)
   T cv[4] = {T(0), T(0), T(0), T(0)};
   if (blockIdx.x < nT * NTRI) {
+    // (out-of-place: the target is read from Lsrc, same layout)
+    const T* Cin = (Lsrc != nullptr) ? Lsrc + (C - L) : C;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int rr = 32 * tr + 16 * wi + Mfma<T>::row(l, r), cc = 32 * tc + 16 * wj + Mfma<T>::col(l, r);
-      cv[r] = C[min(rr, n - 1) * n + min(cc, n - 1)];
+      cv[r] = Cin[min(rr, n - 1) * n + min(cc, n - 1)];
     }
   }
 #pragma unroll
@@ -1405,7 +1416,7 @@ hipError_t init_blocked_kernels() {
 template <typename T, int NB, int NF>
 static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, int prec,
                                   hipStream_t st, hipEvent_t after_potrf, hipEvent_t after_trsm,
-                                  hipEvent_t before_trsm) {
+                                  hipEvent_t before_trsm, const T* Lsrc, const T* Esrc) {
   const int nE = (int)off_diag_storage_len(N);
   const int Lv = recursive_iterations(N);
   const Level lv = level_info(N, lidx);
@@ -1423,9 +1434,10 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
     }
     hipEvent_t ea = nullptr, eb = nullptr;
     if (kt_kernel_events(kKcPotrf, &ea, &eb))
-      hipExtLaunchKernelGGL(k, dim3(lv.dim, B), dim3(potrf_threads(NB)), lds_potrf, st, ea, eb, 0, L, mask, N, n, lv.s);
+      hipExtLaunchKernelGGL(k, dim3(lv.dim, B), dim3(potrf_threads(NB)), lds_potrf, st, ea, eb, 0, L, mask, N, n, lv.s,
+                            Lsrc);
     else
-      hipLaunchKernelGGL(k, dim3(lv.dim, B), dim3(potrf_threads(NB)), lds_potrf, st, L, mask, N, n, lv.s);
+      hipLaunchKernelGGL(k, dim3(lv.dim, B), dim3(potrf_threads(NB)), lds_potrf, st, L, mask, N, n, lv.s, Lsrc);
   }
   if (after_potrf != nullptr) {
     const hipError_t e = hipEventRecord(after_potrf, st);
@@ -1442,11 +1454,11 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
     const int groups = (2 * NB + cpb - 1) / cpb;
     const size_t lds_node = trsm_node_lds<T, NB>();
     hipLaunchKernelGGL((k_trsm_node<T, NB, NF>), dim3(lv.dim, groups, B), dim3(kTrsmThreads), lds_node, st, L, E, mask,
-                       N, n, nE, lv.s, lv.off, cpb);
+                       N, n, nE, lv.s, lv.off, cpb, Esrc);
   }
   else
     hipLaunchKernelGGL((k_trsm_blk<T, NB, NF>), dim3(lv.dim, 2 * NB, B), dim3(trsm_blk_threads(NB)), lds_trsm, st, L, E,
-                       mask, N, n, nE, lv.s, lv.off);
+                       mask, N, n, nE, lv.s, lv.off, Esrc);
   kt_end(kKcTrsm, st);
   if (after_trsm != nullptr) {
     const hipError_t e = hipEventRecord(after_trsm, st);
@@ -1468,13 +1480,13 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
       if constexpr (sizeof(T) == 4)
         if (bf16) k = k_schur_small<T, NB, NF, true>;
       hipLaunchKernelGGL(k, dim3(nsmall, B), dim3(kThreads), lds_small, st, L, E, mask, N, n, nE, lv.s, lv.off,
-                         lv.next_off, nT);
+                         lv.next_off, nT, Lsrc);
     } else {
       auto k = k_schur_blk<T, NF, false>;
       if constexpr (sizeof(T) == 4)
         if (bf16) k = k_schur_blk<T, NF, true>;
       hipLaunchKernelGGL(k, dim3(8 * schur_blk_per_xcd(nT, nF, ntile), B), dim3(kThreads), 0, st, L, E, mask, N, n, nE,
-                         lv.s, lv.off, lv.next_off, nT, nF, ntile);
+                         lv.s, lv.off, lv.next_off, nT, nF, ntile, Lsrc);
     }
     kt_end(kKcSchur, st);
   }
@@ -1511,7 +1523,7 @@ static hipError_t factor_level_large(T* L, T* E, const uint8_t* mask, int B, int
   if (tasks > 0) {
     kt_begin(kKcSchur, st);
     hipLaunchKernelGGL((k_schur_blk<T, 0, false>), dim3(8 * schur_blk_per_xcd(nT, nF, ntile), B), dim3(kThreads), 0, st,
-                       L, E, mask, N, n, nE, lv.s, lv.off, lv.next_off, nT, nF, ntile);
+                       L, E, mask, N, n, nE, lv.s, lv.off, lv.next_off, nT, nF, ntile, (const T*)nullptr);
     kt_end(kKcSchur, st);
   }
   return hipGetLastError();
@@ -1520,21 +1532,32 @@ static hipError_t factor_level_large(T* L, T* E, const uint8_t* mask, int B, int
 template <typename T>
 hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, int prec,
                                 hipStream_t st, hipEvent_t after_potrf, hipEvent_t after_trsm,
-                                hipEvent_t before_trsm) {
-  if (n > kMaxN) return factor_level_large<T>(L, E, mask, B, N, n, lidx, st, after_potrf, after_trsm, before_trsm);
+                                hipEvent_t before_trsm, const T* Lsrc, const T* Esrc) {
+  if (n > kMaxN) {
+    if (Lsrc != nullptr || Esrc != nullptr) return hipErrorInvalidValue;  // in place only
+    return factor_level_large<T>(L, E, mask, B, N, n, lidx, st, after_potrf, after_trsm, before_trsm);
+  }
   const int fp = prec;
   switch (n) {
-    case 32: return factor_level_nb<T, 1, 32>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
-    case 64: return factor_level_nb<T, 2, 64>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
-    case 96: return factor_level_nb<T, 3, 96>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
-    case 128: return factor_level_nb<T, 4, 128>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
+    case 32: return factor_level_nb<T, 1, 32>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm, Lsrc,
+                                                   Esrc);
+    case 64: return factor_level_nb<T, 2, 64>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm, Lsrc,
+                                                   Esrc);
+    case 96: return factor_level_nb<T, 3, 96>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm, Lsrc,
+                                                   Esrc);
+    case 128: return factor_level_nb<T, 4, 128>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm, Lsrc,
+                                                   Esrc);
     default: break;
   }
   switch ((n + 31) / 32) {
-    case 1: return factor_level_nb<T, 1, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
-    case 2: return factor_level_nb<T, 2, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
-    case 3: return factor_level_nb<T, 3, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
-    case 4: return factor_level_nb<T, 4, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm);
+    case 1: return factor_level_nb<T, 1, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm, Lsrc,
+                                                   Esrc);
+    case 2: return factor_level_nb<T, 2, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm, Lsrc,
+                                                   Esrc);
+    case 3: return factor_level_nb<T, 3, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm, Lsrc,
+                                                   Esrc);
+    case 4: return factor_level_nb<T, 4, 0>(L, E, mask, B, N, n, lidx, fp, st, after_potrf, after_trsm, before_trsm, Lsrc,
+                                                   Esrc);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1720,7 +1743,8 @@ void overlap_destroy(SweepOverlap& ov) {
 template <typename T>
 hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, const uint8_t* sweep_mask, int B, int N,
                                int n, int m, int prec, const SweepOverlap& ov, hipStream_t st,
-                               const std::function<hipError_t(hipStream_t)>* side_first) {
+                               const std::function<hipError_t(hipStream_t)>* side_first, const T* Lsrc,
+                               const T* Esrc) {
   const int Lv = recursive_iterations(N);
   hipError_t e = hipEventRecord(ov.fork, st);
   if (e == hipSuccess) e = hipStreamWaitEvent(ov.side, ov.fork, 0);
@@ -1729,7 +1753,8 @@ hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, con
   for (int l = 0; l < Lv && e == hipSuccess; ++l) {
     const bool last = l == Lv - 1;
     e = factor_level_blocked<T>(L, E, factor_mask, B, N, n, l, prec, st, ov.potrf[l],
-                                last ? nullptr : ov.trsm[l], (l == 0 && side_first != nullptr) ? ov.ready : nullptr);
+                                last ? nullptr : ov.trsm[l], (l == 0 && side_first != nullptr) ? ov.ready : nullptr,
+                                l == 0 ? Lsrc : nullptr, l == 0 ? Esrc : nullptr);
     if (e == hipSuccess) e = hipStreamWaitEvent(ov.side, ov.potrf[l], 0);
     if (e == hipSuccess) e = forward_apply_blocked<T>(L, E, x, sweep_mask, B, N, n, m, l, ov.side);
     if (e == hipSuccess && !last) e = hipStreamWaitEvent(ov.side, ov.trsm[l], 0);
@@ -1742,17 +1767,21 @@ hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, con
 
 template hipError_t factor_with_forward<float>(float*, float*, float*, const uint8_t*, const uint8_t*, int, int, int,
                                                int, int, const SweepOverlap&, hipStream_t,
-                                               const std::function<hipError_t(hipStream_t)>*);
+                                               const std::function<hipError_t(hipStream_t)>*, const float*,
+                                               const float*);
 template hipError_t factor_with_forward<double>(double*, double*, double*, const uint8_t*, const uint8_t*, int, int,
                                                 int, int, int, const SweepOverlap&, hipStream_t,
-                                                const std::function<hipError_t(hipStream_t)>*);
+                                                const std::function<hipError_t(hipStream_t)>*, const double*,
+                                                const double*);
 
 template hipError_t trtri_blocked<float>(const float*, float*, const uint8_t*, int, int, int, hipStream_t);
 template hipError_t trtri_blocked<double>(const double*, double*, const uint8_t*, int, int, int, hipStream_t);
 template hipError_t factor_level_blocked<float>(float*, float*, const uint8_t*, int, int, int, int, int,
-                                                hipStream_t, hipEvent_t, hipEvent_t, hipEvent_t);
+                                                hipStream_t, hipEvent_t, hipEvent_t, hipEvent_t, const float*,
+                                                const float*);
 template hipError_t factor_level_blocked<double>(double*, double*, const uint8_t*, int, int, int, int, int,
-                                                 hipStream_t, hipEvent_t, hipEvent_t, hipEvent_t);
+                                                 hipStream_t, hipEvent_t, hipEvent_t, hipEvent_t, const double*,
+                                                 const double*);
 template hipError_t forward_step_blocked<float>(const float*, const float*, float*, const uint8_t*, int, int, int, int,
                                                 int, hipStream_t, bool);
 template hipError_t forward_step_blocked<double>(const double*, const double*, double*, const uint8_t*, int, int, int,

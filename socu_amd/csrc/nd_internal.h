@@ -48,10 +48,14 @@ constexpr int kPrecBf16Schur = 2;
 // after_potrf / after_trsm (optional) are recorded on st after the level's
 // potrf and trsm launches (the forward sweep can follow them on another
 // stream); st waits for before_trsm (optional) between potrf and trsm.
+// Lsrc / Esrc (level 0 only, n <= kMaxN, both or neither): the level reads
+// the input blocks from them instead of L / E and writes its results to
+// L / E -- every L block and every coupling block of E is read first by
+// level 0, so this makes the whole factor out of place.
 template <typename T>
 hipError_t factor_level_blocked(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, int prec,
                                 hipStream_t st, hipEvent_t after_potrf = nullptr, hipEvent_t after_trsm = nullptr,
-                                hipEvent_t before_trsm = nullptr);
+                                hipEvent_t before_trsm = nullptr, const T* Lsrc = nullptr, const T* Esrc = nullptr);
 
 // Per-node kernels for kMaxN < n <= kMaxNLarge (nd_large.hip): potrf with the
 // transposed inverse in the strict upper triangle; trsm of level lidx's R and
@@ -157,7 +161,8 @@ void overlap_destroy(SweepOverlap& ov);
 template <typename T>
 hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, const uint8_t* sweep_mask, int B, int N,
                                int n, int m, int prec, const SweepOverlap& ov, hipStream_t st,
-                               const std::function<hipError_t(hipStream_t)>* side_first = nullptr);
+                               const std::function<hipError_t(hipStream_t)>* side_first = nullptr,
+                               const T* Lsrc = nullptr, const T* Esrc = nullptr);
 
 // Mixed-precision factor_and_solve with FP64 refinement (nd_refine.hip).
 struct MixedPlan {

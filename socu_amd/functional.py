@@ -212,13 +212,15 @@ class _Factor(torch.autograd.Function):
     def forward(L, E):
         lib = _lib.load()
         B, N, n, _ = _dims(L)
-        Lf = L.contiguous().clone()
-        Ef = E.contiguous().clone()
+        # out of place: the factor is written to fresh buffers straight from
+        # the inputs (no copies first)
+        Ls, Es = L.contiguous(), E.contiguous()
+        Lf, Ef = torch.empty_like(Ls), torch.empty_like(Es)
         if B > 0:
             dev = Lf.device
             with torch.cuda.device(dev):
-                _lib.check(lib.socu_cholesky_factor(Lf.data_ptr(), Ef.data_ptr(), None, B, N, n, _code(Lf),
-                                                    _stream(dev)), "cholesky_factor")
+                _lib.check(lib.socu_cholesky_factor_from(Ls.data_ptr(), Es.data_ptr(), Lf.data_ptr(), Ef.data_ptr(),
+                                                         None, B, N, n, _code(Lf), _stream(dev)), "cholesky_factor")
         return Lf, Ef
 
     @staticmethod
@@ -240,14 +242,15 @@ class _FactorAndSolve(torch.autograd.Function):
     def forward(L, E, b):
         lib = _lib.load()
         B, N, n, m = _dims(L, b)
-        Lf = L.contiguous().clone()
-        Ef = E.contiguous().clone()
+        Ls, Es = L.contiguous(), E.contiguous()
+        Lf, Ef = torch.empty_like(Ls), torch.empty_like(Es)
         x = b.contiguous().clone()
         if B > 0:
             dev = Lf.device
             with torch.cuda.device(dev):
-                _lib.check(lib.socu_cholesky_factor_and_solve(Lf.data_ptr(), Ef.data_ptr(), x.data_ptr(), None, B, N,
-                                                              n, m, _code(Lf), _stream(dev)),
+                _lib.check(lib.socu_cholesky_factor_and_solve_from(Ls.data_ptr(), Es.data_ptr(), Lf.data_ptr(),
+                                                                   Ef.data_ptr(), x.data_ptr(), None, B, N, n, m,
+                                                                   _code(Lf), _stream(dev)),
                            "cholesky_factor_and_solve")
         return Lf, Ef, x
 

@@ -69,7 +69,28 @@ def test_uneven_factor_and_solve_matches_oracle(tune, graph, device):
         assert nd.rel_residual(L0[None], E0[None], xs[k].cpu().numpy()[None], b0[None])[0] < 1e-12
     cfg = run.config()
     assert sum(v["systems"] for v in cfg.values()) == len(SHAPES)
-    assert all(v["tile"] in (8, 16, 32) and v["tile"] >= bound and v["waves"] in (1, 4) for bound, v in cfg.items())
+    assert all(v["tile"] in (8, 16, 32) and v["tile"] >= bound and v["waves"] in (1, 2, 4) for bound, v in cfg.items())
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_every_workgroup_size_matches_oracle(waves, device, monkeypatch):
+    """Untuned plans with 1, 2 and 4 wavefronts per workgroup (every bucket),
+    so that each variant of the kernels runs whatever the tuner would pick."""
+    import torch
+    from socu_amd.batched import create_batched_cholesky_launch
+
+    monkeypatch.setenv("SOCU_BATCHED_WAVES", str(waves))
+    systems = _make(SHAPES)
+    Ls, Es, xs = _dev(systems, device, torch.float64)
+    run = create_batched_cholesky_launch(Ls, Es, xs, op="factor_and_solve", tune=False)
+    run()
+    torch.cuda.synchronize()
+    assert all(v["waves"] == waves for v in run.config().values())
+    for k, (L0, E0, b0) in enumerate(systems):
+        Lr, Er, xr = _oracle(L0, E0, b0)
+        assert _rel(np.tril(Ls[k].cpu().numpy()), np.tril(Lr)) < 1e-12, (k, SHAPES[k])
+        assert _rel(Es[k].cpu().numpy(), Er) < 1e-12, (k, SHAPES[k])
+        assert _rel(xs[k].cpu().numpy(), xr) < 1e-12, (k, SHAPES[k])
 
 
 def test_graph_replays_match_eager(device):

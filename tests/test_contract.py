@@ -293,3 +293,51 @@ def test_solve_with_stored_inverses_equals_solve(n, N, m, dtype, device):
     torch.cuda.synchronize(device)
     a, b = x1.cpu().numpy(), x2.cpu().numpy()
     assert np.linalg.norm(b - a) / np.linalg.norm(a) < (1e-13 if dtype == "float64" else 1e-5)
+
+
+def _capi_codes(dtype):
+    from socu_amd import _lib
+    return _lib.load(), (_lib.DTYPE_F64 if dtype == "float64" else _lib.DTYPE_F32)
+
+
+@pytest.mark.parametrize("n,N,m,dtype", [(128, 24, 1, "float64"), (40, 33, 2, "float64"), (64, 50, 1, "float32"),
+                                         (32, 1, 1, "float64"), (8, 2, 3, "float32"), (160, 9, 2, "float64")])
+def test_out_of_place_factor_equals_in_place(n, N, m, dtype, device):
+    """socu_cholesky_factor_from / _and_solve_from: bit-identical to the
+    in-place entry points on copies of the inputs (the same kernels read the
+    first level's blocks from the sources), the sources untouched, masked
+    systems' outputs not written.  n = 160 takes the copy-then-factor path."""
+    import torch
+
+    lib, code = _capi_codes(dtype)
+    tdt = getattr(torch, dtype)
+    B = 3
+    L0, E0, b0 = _stack(n, N, m, B, seed=71, alpha=32.0 if n > 128 else 16.0)
+    st = torch.cuda.current_stream(device).cuda_stream
+    mask = torch.tensor([1, 0, 1], dtype=torch.uint8, device=device)
+    for with_solve in (False, True):
+        Ls = torch.tensor(L0, dtype=tdt, device=device)
+        Es = torch.tensor(E0, dtype=tdt, device=device)
+        Li, Ei, xi = Ls.clone(), Es.clone(), torch.tensor(b0, dtype=tdt, device=device)
+        Lo = torch.full_like(Ls, 7.0)
+        Eo = torch.full_like(Es, 7.0)
+        xo = xi.clone()
+        if with_solve:
+            assert lib.socu_cholesky_factor_and_solve(Li.data_ptr(), Ei.data_ptr(), xi.data_ptr(), mask.data_ptr(), B,
+                                                      N, n, m, code, st) == 0
+            assert lib.socu_cholesky_factor_and_solve_from(Ls.data_ptr(), Es.data_ptr(), Lo.data_ptr(), Eo.data_ptr(),
+                                                           xo.data_ptr(), mask.data_ptr(), B, N, n, m, code, st) == 0
+        else:
+            assert lib.socu_cholesky_factor(Li.data_ptr(), Ei.data_ptr(), mask.data_ptr(), B, N, n, code, st) == 0
+            assert lib.socu_cholesky_factor_from(Ls.data_ptr(), Es.data_ptr(), Lo.data_ptr(), Eo.data_ptr(),
+                                                 mask.data_ptr(), B, N, n, code, st) == 0
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(Ls.cpu().numpy(), np.asarray(L0, dtype=dtype))
+        np.testing.assert_array_equal(Es.cpu().numpy(), np.asarray(E0, dtype=dtype))
+        for b in (0, 2):
+            np.testing.assert_array_equal(Lo[b].cpu().numpy(), Li[b].cpu().numpy())
+            np.testing.assert_array_equal(Eo[b].cpu().numpy(), Ei[b].cpu().numpy())
+            np.testing.assert_array_equal(xo[b].cpu().numpy(), xi[b].cpu().numpy())
+        if n <= 128:  # masked system: outputs not written
+            assert bool((Lo[1] == 7.0).all()) and bool((Eo[1] == 7.0).all())
+        np.testing.assert_array_equal(xo[1].cpu().numpy(), np.asarray(b0[1], dtype=dtype))

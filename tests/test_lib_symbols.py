@@ -86,3 +86,27 @@ def test_spin_limit_setter_is_host_only():
     lib = _lib.load()
     assert lib.socu_set_sweep_spin_limit(-1) == 0
     assert lib.socu_set_sweep_spin_limit(0) == 0
+
+
+def test_out_of_place_factor_rejects_overlapping_or_missing_sources():
+    """The out-of-place factor entry points check their sources on the host
+    (before any device work): missing sources and sources that overlap the
+    outputs are EINVAL."""
+    lib = _lib.load()
+    f64 = _lib.DTYPE_F64
+    B, N, n = 2, 5, 4
+    nL = B * N * n * n * 8
+    nE = B * lib.socu_calculate_off_diag_storage_len(N) * n * n * 8
+    L, E = 1 << 32, (1 << 32) + (1 << 24)  # placeholder addresses: never dereferenced on these paths
+    Ls, Es = (1 << 33), (1 << 33) + (1 << 24)
+    assert nL < (1 << 24) and nE < (1 << 24)
+    rc = lib.socu_cholesky_factor_from(None, Es, L, E, None, B, N, n, f64, None)
+    assert rc == _lib.EINVAL and "L_src" in _lib.last_error()
+    rc = lib.socu_cholesky_factor_from(Ls, None, L, E, None, B, N, n, f64, None)
+    assert rc == _lib.EINVAL and "E_src" in _lib.last_error()
+    rc = lib.socu_cholesky_factor_from(L + 8, Es, L, E, None, B, N, n, f64, None)
+    assert rc == _lib.EINVAL and "overlap" in _lib.last_error()
+    rc = lib.socu_cholesky_factor_and_solve_from(Ls, E + nE - 8, L, E, Ls + (1 << 20), None, B, N, n, 1, f64, None)
+    assert rc == _lib.EINVAL and "overlap" in _lib.last_error()
+    rc = lib.socu_cholesky_factor_and_solve_from(Ls, Es, L, E, None, None, B, N, n, 1, f64, None)
+    assert rc == _lib.EINVAL and "x" in _lib.last_error()
