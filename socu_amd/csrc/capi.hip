@@ -950,7 +950,8 @@ int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) 
              h->op, b, h->counts[b], h->bcost[b]);
     keys[b] = buf;
     auto it = g_tune.find(keys[b]);
-    if (it != g_tune.end() && (it->second & 7) >= b) {
+    if (it != g_tune.end() && (it->second & 7) >= b &&
+        socu::batched_fits(it->second & 7, it->second >> 3, h->mmax[b], h->dtype)) {
       h->tile[b] = it->second & 7;
       h->waves[b] = it->second >> 3;
     } else {
@@ -992,11 +993,15 @@ int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) 
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   for (int b = 0; b < kClasses && e == hipSuccess; ++b) {
-    if (h->counts[b] == 0 || (g_tune.count(keys[b]) && (g_tune[keys[b]] & 7) >= b)) continue;
+    if (h->counts[b] == 0 ||
+        (g_tune.count(keys[b]) && (g_tune[keys[b]] & 7) >= b &&
+         socu::batched_fits(g_tune[keys[b]] & 7, g_tune[keys[b]] >> 3, h->mmax[b], h->dtype)))
+      continue;
     float best = 1e30f;
     int best_c = b, best_w = 1;
     for (int c = b; c < kClasses && e == hipSuccess; ++c) {
       for (int w : {1, 2, 4}) {
+        if (!socu::batched_fits(c, w, h->mmax[b], h->dtype)) continue;
         for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
           e = socu::batched_copy(h->dprobs, dcopy, h->count, h->op, h->dtype, ts);
           if (e == hipSuccess) e = hipEventRecord(e0, ts);
@@ -1099,10 +1104,15 @@ int socu_batched_create(socu_batched** out, int op, const socu_batch_system* sys
   if (rc == SOCU_OK && !tune) {
     // SOCU_BATCHED_WAVES=1|2|4 sets the wavefronts per workgroup of an
     // untuned plan (every variant of the kernels can then be run on purpose)
+    // (or the widest that fits the bucket's right-hand sides in LDS)
     const char* w = getenv("SOCU_BATCHED_WAVES");
     const int wv = w ? atoi(w) : 1;
     if (wv == 1 || wv == 2 || wv == 4)
-      for (int b = 0; b < kClasses; ++b) h->waves[b] = wv;
+      for (int b = 0; b < kClasses; ++b) {
+        int v = wv;
+        while (v > 1 && !socu::batched_fits(h->tile[b], v, h->mmax[b], h->dtype)) v /= 2;
+        h->waves[b] = v;
+      }
   }
   if (rc == SOCU_OK && tune) rc = batched_tune(h, probs);
   if (rc) {

@@ -49,14 +49,17 @@ struct BSlot {
 };
 
 // Workgroups of W wavefronts.  A task uses LPT = min(64 W, NT^2/4) lanes
-// (at most four tile elements per lane) and the workgroup runs TPW tasks at
-// once: NT = 32: one task (W = 1: one wavefront does everything); NT = 16:
-// one task per wavefront; NT = 8: four tasks per wavefront.
+// (at most four tile elements per lane; NT = 32: 64 lanes, sixteen elements)
+// and the workgroup runs TPW tasks at once: NT = 32 and NT = 16: one task per
+// wavefront; NT = 8: four tasks per wavefront.
 template <int NT, int W>
 struct BShape {
   static constexpr int TP = NT + 1;
   static constexpr int BT = 64 * W;                  // threads per workgroup
-  static constexpr int NAT = NT * NT / 4;            // lanes at 4 outputs per lane
+  // lanes a task needs at 4 outputs per lane; NT = 32: one wavefront per
+  // task (16 per lane), so that wider workgroups run several tasks of the
+  // system at once instead of spreading one task (measured slower)
+  static constexpr int NAT = NT == 32 ? 64 : NT * NT / 4;
   static constexpr int LPT = NAT < BT ? NAT : BT;    // lanes per task
   static constexpr int TPW = BT / LPT;               // tasks per workgroup
   static constexpr int EPL = NT * NT / LPT;          // tile elements per lane
@@ -255,8 +258,9 @@ __device__ __forceinline__ void vprod(int n, int m, int kend, FA fa, FX fx, bool
 template <typename T, int NT, int W>
 __device__ __forceinline__ void slots_chol(BSlot<T, NT>* slots, bool node) {
   constexpr int TP = BShape<NT, W>::TP;
-  if constexpr (NT == 32) {  // one task per workgroup (node is uniform); its first wavefront
-    if (node && wave_id() == 0) tile_chol_inv32b<T>(slots[0].A, TP, slots[0].dinv, slots[0].col);
+  if constexpr (NT == 32) {  // one task per wavefront (node is wavefront-uniform)
+    BSlot<T, NT>& S = slots[wave_id()];
+    if (node) tile_chol_inv32b<T>(S.A, TP, S.dinv, S.col);
   } else if constexpr (NT == 16) {  // one task per wavefront (node is wavefront-uniform)
     BSlot<T, NT>& S = slots[wave_id()];
     if (node && lane_id() < 32) tile_chol_inv16<T>(S.A, TP, S.dinv, S.col);
@@ -563,10 +567,21 @@ size_t lds_bytes(int yp) {
   return BShape<NT, W>::TPW * (sizeof(BSlot<T, NT>) + (size_t)3 * NT * yp * sizeof(T));
 }
 
+constexpr size_t kLdsPerCu = 160 * 1024;
+
+// (NT = 32, FP64, four wavefronts: four tasks' tiles leave room for m <= 7)
 template <typename T, int NT, int W>
 hipError_t set_attr() {
+  const size_t lds = lds_bytes<T, NT, W>(NT + 1);
   return hipFuncSetAttribute((const void*)k_batched<T, NT, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)lds_bytes<T, NT, W>(NT + 1));
+                             (int)(lds < kLdsPerCu ? lds : kLdsPerCu));
+}
+
+template <typename T, int NT>
+bool fits_nt(int waves, int mmax) {
+  const int yp = mmax + 1;
+  const size_t lds = waves == 1 ? lds_bytes<T, NT, 1>(yp) : (waves == 2 ? lds_bytes<T, NT, 2>(yp) : lds_bytes<T, NT, 4>(yp));
+  return lds <= kLdsPerCu;
 }
 
 template <typename T, int NT>
@@ -620,10 +635,17 @@ hipError_t init_batched_kernels() {
 
 int batched_class(int n) { return n <= 8 ? 0 : (n <= 16 ? 1 : 2); }
 
+bool batched_fits(int cls, int waves, int mmax, int dtype) {
+  if (cls < 0 || cls > 2 || (waves != 1 && waves != 2 && waves != 4)) return false;
+  if (dtype == SOCU_DTYPE_F64)
+    return cls == 0 ? fits_nt<double, 8>(waves, mmax) : (cls == 1 ? fits_nt<double, 16>(waves, mmax) : fits_nt<double, 32>(waves, mmax));
+  return cls == 0 ? fits_nt<float, 8>(waves, mmax) : (cls == 1 ? fits_nt<float, 16>(waves, mmax) : fits_nt<float, 32>(waves, mmax));
+}
+
 hipError_t batched_launch_class(const BatchProblem* dprobs, const int* dorder, int cls, int count, int mmax, int waves,
                                 const uint8_t* mask, int op, int dtype, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  if (cls < 0 || cls > 2 || mmax < 1 || mmax > (8 << cls) || (waves != 1 && waves != 2 && waves != 4))
+  if (cls < 0 || cls > 2 || mmax < 1 || mmax > (8 << cls) || !batched_fits(cls, waves, mmax, dtype))
     return hipErrorInvalidValue;
   if (dtype == SOCU_DTYPE_F64) {
     if (cls == 0) launch_nt<double, 8>(dprobs, dorder, count, mmax, waves, mask, op, st);

@@ -208,8 +208,10 @@ hipError_t init_batched_kernels();
 // Tile class of a block size: 0 (tiles of 8), 1 (16), 2 (32).
 int batched_class(int n);
 // One launch over `count` systems with tile class `cls` and workgroups of
-// `waves` (1 or 4) wavefronts, listed (by index into dprobs) at dorder; mmax
-// is the largest m among them (<= 8 << cls).
+// `waves` (1, 2 or 4) wavefronts, listed (by index into dprobs) at dorder;
+// mmax is the largest m among them (<= 8 << cls).  Fails when that
+// workgroup's LDS would exceed a compute unit's (batched_fits).
+bool batched_fits(int cls, int waves, int mmax, int dtype);
 hipError_t batched_launch_class(const BatchProblem* dprobs, const int* dorder, int cls, int count, int mmax, int waves,
                                 const uint8_t* mask, int op, int dtype, hipStream_t st);
 // Copies L, E (and x unless op is a factor) of count systems src[k] -> dst[k].

@@ -72,25 +72,36 @@ def test_uneven_factor_and_solve_matches_oracle(tune, graph, device):
     assert all(v["tile"] in (8, 16, 32) and v["tile"] >= bound and v["waves"] in (1, 2, 4) for bound, v in cfg.items())
 
 
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
 @pytest.mark.parametrize("waves", [1, 2, 4])
-def test_every_workgroup_size_matches_oracle(waves, device, monkeypatch):
+def test_every_workgroup_size_matches_oracle(waves, dtype, device, monkeypatch):
     """Untuned plans with 1, 2 and 4 wavefronts per workgroup (every bucket),
-    so that each variant of the kernels runs whatever the tuner would pick."""
+    so that each variant of the kernels runs whatever the tuner would pick.
+    Workgroups whose LDS would not fit fall back to fewer wavefronts (FP64
+    tiles of 32 with four tasks per workgroup leave room for m <= 7 only):
+    the full shape list, then the same shapes with m <= 4, where every bucket
+    takes the requested width."""
     import torch
     from socu_amd.batched import create_batched_cholesky_launch
 
     monkeypatch.setenv("SOCU_BATCHED_WAVES", str(waves))
-    systems = _make(SHAPES)
-    Ls, Es, xs = _dev(systems, device, torch.float64)
-    run = create_batched_cholesky_launch(Ls, Es, xs, op="factor_and_solve", tune=False)
-    run()
-    torch.cuda.synchronize()
-    assert all(v["waves"] == waves for v in run.config().values())
-    for k, (L0, E0, b0) in enumerate(systems):
-        Lr, Er, xr = _oracle(L0, E0, b0)
-        assert _rel(np.tril(Ls[k].cpu().numpy()), np.tril(Lr)) < 1e-12, (k, SHAPES[k])
-        assert _rel(Es[k].cpu().numpy(), Er) < 1e-12, (k, SHAPES[k])
-        assert _rel(xs[k].cpu().numpy(), xr) < 1e-12, (k, SHAPES[k])
+    tdt = getattr(torch, dtype)
+    tol = 1e-12 if dtype == "float64" else 2e-4
+    for shapes in (SHAPES, [(N, n, min(m, 4)) for N, n, m in SHAPES]):
+        systems = _make(shapes)
+        Ls, Es, xs = _dev(systems, device, tdt)
+        run = create_batched_cholesky_launch(Ls, Es, xs, op="factor_and_solve", tune=False, dtype=tdt)
+        run()
+        torch.cuda.synchronize()
+        cfg = run.config()
+        assert all(v["waves"] <= waves for v in cfg.values())
+        if max(m for _, _, m in shapes) <= 4:
+            assert all(v["waves"] == waves for v in cfg.values()), cfg
+        for k, (L0, E0, b0) in enumerate(systems):
+            Lr, Er, xr = _oracle(L0, E0, b0)
+            assert _rel(np.tril(Ls[k].double().cpu().numpy()), np.tril(Lr)) < tol, (k, shapes[k])
+            assert _rel(Es[k].double().cpu().numpy(), Er) < tol, (k, shapes[k])
+            assert _rel(xs[k].double().cpu().numpy(), xr) < tol, (k, shapes[k])
 
 
 def test_graph_replays_match_eager(device):
