@@ -1031,7 +1031,10 @@ __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* 
 // The updates spread each block over NB workgroups (one per 32 rows) so that
 // the E blocks stream through many CUs; the applies are triangular GEMVs.
 // ---------------------------------------------------------------------------
-constexpr int kRP = kRhsChunk + 1;  // pitch of right-hand-side panels in LDS
+// Right-hand-side panels in LDS have pitch yp = min(m, kRhsChunk) + 1
+// (dynamic shared memory sized by the launch: few right-hand sides leave room
+// for more workgroups per compute unit).
+__host__ __device__ constexpr int rhs_pitch(int m) { return (m < kRhsChunk ? m : kRhsChunk) + 1; }
 
 // One 32-row block of the update.  "Row operand" A (rows r0..r0+31 of an
 // n x n block, times Ya) and "column operand" C (columns r0..r0+31, i.e.
@@ -1040,6 +1043,7 @@ template <typename T, int NB, int NF, bool BWD>
 __global__ __launch_bounds__(kThreads) void k_sweep_update(const T* __restrict__ E, T* __restrict__ x,
                                                           const uint8_t* __restrict__ mask, int N, int n_, int m,
                                                           int nE, int s, int off) {
+  const int yp = rhs_pitch(m);
   const int n = NF ? NF : n_;
   constexpr int NP = 32 * NB;
   constexpr int AP = NP + 1;
@@ -1078,8 +1082,9 @@ __global__ __launch_bounds__(kThreads) void k_sweep_update(const T* __restrict__
     if (ja < 0 && jc < 0) return;
   }
 
-  __shared__ T Ya[NP * kRP];
-  __shared__ T Yc[NP * kRP];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Ya = reinterpret_cast<T*>(smem_raw);  // NP x yp
+  T* Yc = Ya + NP * yp;                     // NP x yp
   __shared__ T As[32 * AP];
   __shared__ T Cs[NP * 33];
   const int tid = threadIdx.x;
@@ -1098,8 +1103,8 @@ __global__ __launch_bounds__(kThreads) void k_sweep_update(const T* __restrict__
   }
   for (int q = tid; q < NP * w; q += kThreads) {
     const int k = q / w, c = q - k * w;
-    Ya[k * kRP + c] = (ja >= 0 && k < n) ? xb[(size_t)ja * nm + (size_t)k * m + c0 + c] : T(0);
-    Yc[k * kRP + c] = (jc >= 0 && k < n) ? xb[(size_t)jc * nm + (size_t)k * m + c0 + c] : T(0);
+    Ya[k * yp + c] = (ja >= 0 && k < n) ? xb[(size_t)ja * nm + (size_t)k * m + c0 + c] : T(0);
+    Yc[k * yp + c] = (jc >= 0 && k < n) ? xb[(size_t)jc * nm + (size_t)k * m + c0 + c] : T(0);
   }
 #pragma unroll
   for (int e = 0; e < PA; ++e) {
@@ -1119,7 +1124,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_update(const T* __restrict__
 #pragma unroll
     for (int e = 0; e < KP; ++e) {
       const int k = part * KP + e;
-      acc += As[r * AP + k] * Ya[k * kRP + c] + Cs[k * 33 + r] * Yc[k * kRP + c];
+      acc += As[r * AP + k] * Ya[k * yp + c] + Cs[k * 33 + r] * Yc[k * yp + c];
     }
 #pragma unroll
     for (int o = 4; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 8);
@@ -1149,8 +1154,10 @@ __global__ __launch_bounds__(kThreads) void k_apply_fwd(const T* __restrict__ L,
   const size_t nn = (size_t)n * n, nm = (size_t)n * m;
   const T* U = L + ((size_t)b * N + j) * nn;
   T* y = x + ((size_t)b * N + j) * nm;
-  __shared__ T Y[NP * kRP];
-  __shared__ T Pt[PARTS * NP * kRP];
+  const int yp = rhs_pitch(m);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Y = reinterpret_cast<T*>(smem_raw);  // NP x yp
+  T* Pt = Y + NP * yp;                     // PARTS x NP x yp
   const int tid = threadIdx.x;
   const int r = tid % NP, part = tid / NP;
   const bool act = part < PARTS;
@@ -1172,7 +1179,7 @@ __global__ __launch_bounds__(kThreads) void k_apply_fwd(const T* __restrict__ L,
   }
   for (int q = tid; q < NP * w; q += kThreads) {
     const int k = q / w, c = q - k * w;
-    Y[k * kRP + c] = (k < n) ? y[(size_t)k * m + c0 + c] : T(0);
+    Y[k * yp + c] = (k < n) ? y[(size_t)k * m + c0 + c] : T(0);
   }
   __syncthreads();
   if (act) {
@@ -1181,16 +1188,16 @@ __global__ __launch_bounds__(kThreads) void k_apply_fwd(const T* __restrict__ L,
 #pragma unroll
       for (int e = 0; e < KP; ++e) {
         const int k = part * KP + e;
-        if (k < NP) acc += a[e] * Y[k * kRP + c];
+        if (k < NP) acc += a[e] * Y[k * yp + c];
       }
-      Pt[(part * NP + r) * kRP + c] = acc;
+      Pt[(part * NP + r) * yp + c] = acc;
     }
   }
   __syncthreads();
   if (part == 0 && r < n) {
     for (int c = 0; c < w; ++c) {
-      T acc = Y[r * kRP + c] / dg;
-      for (int p = 0; p < PARTS; ++p) acc += Pt[(p * NP + r) * kRP + c];
+      T acc = Y[r * yp + c] / dg;
+      for (int p = 0; p < PARTS; ++p) acc += Pt[(p * NP + r) * yp + c];
       y[(size_t)r * m + c0 + c] = acc;
     }
   }
@@ -1215,7 +1222,9 @@ __global__ __launch_bounds__(kThreads) void k_apply_bwd(const T* __restrict__ L,
   const size_t nn = (size_t)n * n, nm = (size_t)n * m;
   const T* U = L + ((size_t)b * N + i) * nn;
   T* y = x + ((size_t)b * N + i) * nm;
-  __shared__ T Y[NP * kRP];
+  const int yp = rhs_pitch(m);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Y = reinterpret_cast<T*>(smem_raw);  // NP x yp
   const int tid = threadIdx.x;
   const int sub = tid & 7, rr = tid >> 3;
   T a[PASSES][KPL];
@@ -1238,7 +1247,7 @@ __global__ __launch_bounds__(kThreads) void k_apply_bwd(const T* __restrict__ L,
   }
   for (int q = tid; q < NP * w; q += kThreads) {
     const int k = q / w, c = q - k * w;
-    Y[k * kRP + c] = (k < n) ? y[(size_t)k * m + c0 + c] : T(0);
+    Y[k * yp + c] = (k < n) ? y[(size_t)k * m + c0 + c] : T(0);
   }
   __syncthreads();
 #pragma unroll
@@ -1248,10 +1257,10 @@ __global__ __launch_bounds__(kThreads) void k_apply_bwd(const T* __restrict__ L,
     for (int c = 0; c < w; ++c) {
       T acc = T(0);
 #pragma unroll
-      for (int e = 0; e < KPL; ++e) acc += a[p][e] * Y[(sub * KPL + e) * kRP + c];
+      for (int e = 0; e < KPL; ++e) acc += a[p][e] * Y[(sub * KPL + e) * yp + c];
 #pragma unroll
       for (int o = 4; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 8);
-      if (sub == 0 && r < n) y[(size_t)r * m + c0 + c] = acc + Y[r * kRP + c] / dg;
+      if (sub == 0 && r < n) y[(size_t)r * m + c0 + c] = acc + Y[r * yp + c] / dg;
     }
   }
 }
@@ -1259,6 +1268,20 @@ __global__ __launch_bounds__(kThreads) void k_apply_bwd(const T* __restrict__ L,
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// dynamic LDS of the sweep kernels for m right-hand sides
+template <typename T, int NB>
+constexpr size_t update_lds(int m) {
+  return sizeof(T) * 2 * (size_t)(32 * NB) * rhs_pitch(m);
+}
+template <typename T, int NB>
+constexpr size_t apply_fwd_lds(int m) {
+  return sizeof(T) * (1 + kThreads / (32 * NB)) * (size_t)(32 * NB) * rhs_pitch(m);
+}
+template <typename T, int NB>
+constexpr size_t apply_bwd_lds(int m) {
+  return sizeof(T) * (size_t)(32 * NB) * rhs_pitch(m);
+}
+
 template <typename T, int NB>
 constexpr size_t potrf_blk_lds() {
   return sizeof(T) * ((size_t)(32 * NB) * (32 * NB + 1) + 32 * NB + 128 + 32 * 33);
@@ -1606,14 +1629,18 @@ static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* m
   if (update && step >= 0) {
     const Level lv = level_info(N, step);
     const int items = N / (2 * lv.s);
-    if (items > 0)
-      hipLaunchKernelGGL((k_sweep_update<T, NB, NF, false>), dim3(items, NB * mch, B), dim3(kThreads), 0, st, E,
-                         x, mask, N, n, m, nE, lv.s, lv.off);
+    if (items > 0) {
+      const size_t lds = update_lds<T, NB>(m);
+      hipLaunchKernelGGL((k_sweep_update<T, NB, NF, false>), dim3(items, NB * mch, B), dim3(kThreads), lds, st, E, x,
+                         mask, N, n, m, nE, lv.s, lv.off);
+    }
   }
   const Level nx = level_info(N, step + 1);
-  if (apply_next && nx.dim > 0)
-    hipLaunchKernelGGL((k_apply_fwd<T, NB, NF>), dim3(nx.dim, mch, B), dim3(kThreads), 0, st, L, x, mask, N, n, m,
+  if (apply_next && nx.dim > 0) {
+    const size_t lds = apply_fwd_lds<T, NB>(m);
+    hipLaunchKernelGGL((k_apply_fwd<T, NB, NF>), dim3(nx.dim, mch, B), dim3(kThreads), lds, st, L, x, mask, N, n, m,
                        nx.s);
+  }
   kt_end(kKcForward, st);
   return hipGetLastError();
 }
@@ -1672,11 +1699,14 @@ static hipError_t backward_level_nb(const T* L, const T* E, T* x, const uint8_t*
   if (lv.dim <= 0 || B <= 0 || mch <= 0) return hipSuccess;
   kt_begin(kKcBackward, st);
   // the top node has no neighbours (i = s-1, i+s >= N)
-  if (!(lv.dim == 1 && lv.s - 1 + lv.s >= N))
-    hipLaunchKernelGGL((k_sweep_update<T, NB, NF, true>), dim3(lv.dim, NB * mch, B), dim3(kThreads), 0, st, E, x,
+  if (!(lv.dim == 1 && lv.s - 1 + lv.s >= N)) {
+    const size_t lds = update_lds<T, NB>(m);
+    hipLaunchKernelGGL((k_sweep_update<T, NB, NF, true>), dim3(lv.dim, NB * mch, B), dim3(kThreads), lds, st, E, x,
                        mask, N, n, m, nE, lv.s, lv.off);
-  hipLaunchKernelGGL((k_apply_bwd<T, NB, NF>), dim3(lv.dim, mch, B), dim3(kThreads), 0, st, L, x, mask, N, n, m,
-                     lv.s);
+  }
+  const size_t lds_apply = apply_bwd_lds<T, NB>(m);
+  hipLaunchKernelGGL((k_apply_bwd<T, NB, NF>), dim3(lv.dim, mch, B), dim3(kThreads), lds_apply, st, L, x, mask, N, n,
+                     m, lv.s);
   kt_end(kKcBackward, st);
   return hipGetLastError();
 }
