@@ -265,9 +265,13 @@ def bench_adjoint(args, dev, st):
     per, _ = _time_steps(step, lambda: None, st, args.secondary_steps, 3)
     perf, _ = _time_steps(fwd, lambda: None, st, args.secondary_steps, 3)
     ms, msf = float(per.mean()), float(perf.mean())
+    # the forward is one batched factor_and_solve: its compulsory bytes and
+    # flops are B times those of one system
+    ff, fsv = problems.nd_flops(N, n, m)
     res = {"workload": f"B={B} n={n} N={N} m={m} FP64, forward + backward (dL, dE, db)",
            "ms_per_step": round(ms, 4), "forward_ms": round(msf, 4),
-           "backward_ms": round(ms - msf, 4), "systems_per_s": round(B / (ms * 1e-3), 1)}
+           "backward_ms": round(ms - msf, 4), "systems_per_s": round(B / (ms * 1e-3), 1),
+           "forward_path": path_fractions(B * (ff + fsv), B * problems.nd_bytes(N, n, m)[2], msf)}
     log(f"bench: adjoint {res}")
     return res
 

@@ -279,6 +279,114 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
 }
 
 // ---------------------------------------------------------------------------
+// One whole level-l node step for n == 32, by one wavefront per node:
+//   L_i <- chol(L_i) with the transposed inverse in its strict upper triangle
+//          (as k_potrf_blk),
+//   R_i <- R_i Linv^T,  Lc_i <- Linv Lc_i          (as k_trsm_node),
+//   F   <- -R_i Lc_i    the fill-in block (i+s, i-s) of the next level (the
+//                       fill-in tasks of k_schur_small / k_schur_blk).
+// The reference runs these as separate potrf, trsm and gemm passes over the
+// level (socu/blas/potrf.py:97-119, socu/blas/trsm.py:94-120 and :202-228,
+// block_tridiag_solver.py:797-813).  At n = 32 the pivot chain of the
+// diagonal block is latency-bound and uses little of a compute unit, so the
+// coupling blocks' loads, MFMA products and stores of one node run while the
+// other wavefronts on the same SIMD are in their chains; R_i and Lc_i are
+// read once (the separate kernels read them for the trsm and again for the
+// fill-in) and the factor is not re-read for the trsm.
+// R_i = block (i+s, i) exists when i + s < N, Lc_i = block (i, i-s) when
+// i >= s, F when both do.  Layouts and storage as the separate kernels;
+// Lsrc / Esrc (optional) are the out-of-place sources of L_i, R_i and Lc_i.
+// ---------------------------------------------------------------------------
+constexpr size_t node32_lds(size_t esize) { return esize * (2 * 32 * 33 + 32 + 128); }
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_node32(T* __restrict__ L, T* __restrict__ E, const uint8_t* __restrict__ mask,
+                                              int N, int nE, int s, int off, int next_off,
+                                              const T* __restrict__ Lsrc, const T* __restrict__ Esrc) {
+  constexpr int n = 32, P = 33;
+  constexpr size_t nn = 32 * 32;
+  const int b = blockIdx.y;
+  if (masked_out(mask, b)) return;
+  const int i = s - 1 + 2 * s * blockIdx.x;
+  const bool hasR = i + s < N, hasC = i >= s;
+  T* A = L + ((size_t)b * N + i) * nn;
+  const T* Ain = (Lsrc != nullptr) ? Lsrc + (A - L) : A;
+  T* Eb = E + (size_t)b * nE * nn;
+  T* XR = hasR ? Eb + (size_t)(off + i / s) * nn : nullptr;        // block (i+s, i)
+  T* XC = hasC ? Eb + (size_t)(off + (i - s) / s) * nn : nullptr;  // block (i, i-s)
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* S = reinterpret_cast<T*>(smem_raw);  // 32 x 33: the block, then R_i, Lc_i, R_i'
+  T* ut = S + 32 * P;                       // 32 x 33: Linv transposed, then Lc_i'
+  T* dinv = ut + 32 * P;
+  T* col = dinv + 32;
+  const int l = lane_id();
+  const int lr = l >> 5, lc = l & 31;  // lane l moves row 2q + lr, column lc of a block
+
+  {
+    T v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = Ain[(2 * q + lr) * n + lc];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int r = 2 * q + lr;
+      S[r * P + lc] = (lc <= r) ? v[q] : T(0);
+    }
+  }
+  // R_i and Lc_i are loaded late in the pivot chain (pair step 20), when the
+  // chain's column registers of the early steps are free, and land while the
+  // chain finishes (earlier, the kernel needs more than the 256 registers
+  // that let two wavefronts share a SIMD).
+  T rv[16], cv[16];
+  const T* XRin = hasR ? ((Esrc != nullptr) ? Esrc + (XR - E) : XR) : nullptr;
+  const T* XCin = hasC ? ((Esrc != nullptr) ? Esrc + (XC - E) : XC) : nullptr;
+  auto hook = [&](int k) {
+    if (k == 20) {
+      if (hasR) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) rv[q] = XRin[(2 * q + lr) * n + lc];
+      }
+      if (hasC) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) cv[q] = XCin[(2 * q + lr) * n + lc];
+      }
+    }
+  };
+  tile_chol_inv<T>(S, P, dinv, col, ut, hook);
+  // the factor (lower) with the transposed inverse (strict upper)
+#pragma unroll
+  for (int q = 0; q < 16; ++q) A[(2 * q + lr) * n + lc] = S[(2 * q + lr) * P + lc];
+  if (!hasR && !hasC) return;
+
+  // (LDS operations of one wavefront complete in issue order: S and ut are
+  // rewritten below only after the reads of their previous contents.)
+  typename Mfma<T>::acc_t ar[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+  typename Mfma<T>::acc_t ac[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+  if (hasR) {
+    // R' = R Linv^T: B(k, c) = Linv[c][k] = ut[k][c]
+#pragma unroll
+    for (int q = 0; q < 16; ++q) S[(2 * q + lr) * P + lc] = rv[q];
+    wave_gemm<T>(ar, S, P, 1, ut, P, 1, 32);
+    acc_foreach<T>(ar, [&](int r, int c, T v) { XR[r * n + c] = v; });
+  }
+  if (hasC) {
+    // Lc' = Linv Lc: A(r, k) = Linv[r][k] = ut[k][r]
+#pragma unroll
+    for (int q = 0; q < 16; ++q) S[(2 * q + lr) * P + lc] = cv[q];
+    wave_gemm<T>(ac, ut, 1, P, S, P, 1, 32);
+    acc_foreach<T>(ac, [&](int r, int c, T v) { XC[r * n + c] = v; });
+  }
+  if (hasR && hasC) {
+    // F = -R' Lc' from R' in S and Lc' in ut
+    T* F = Eb + (size_t)(next_off + (i - s) / (2 * s)) * nn;
+    acc_foreach<T>(ar, [&](int r, int c, T v) { S[r * P + c] = v; });
+    acc_foreach<T>(ac, [&](int r, int c, T v) { ut[r * P + c] = v; });
+    typename Mfma<T>::acc_t af[2][2] = {{acc_zero<T>(), acc_zero<T>()}, {acc_zero<T>(), acc_zero<T>()}};
+    wave_gemm<T>(af, S, P, 1, ut, P, 1, 32);
+    acc_foreach<T>(af, [&](int r, int c, T v) { F[r * n + c] = -v; });
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Inverses of the diagonal factor blocks from tril(L) alone, for solves whose
 // factor may come from anywhere (the reference's sweeps read only tril(L),
 // socu/blas/potrf.py:101-103).  One workgroup of NB wavefronts per block
@@ -1446,7 +1554,26 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
   if (lv.dim <= 0 || B <= 0) return hipSuccess;
   const size_t lds_trsm = trsm_blk_lds<T, NB>();
   const bool bf16 = (prec & kPrecBf16Schur) != 0 && sizeof(T) == 4;
-  {
+  // n == 32 in the storage precision: one fused node kernel (k_node32) per
+  // level instead of potrf, trsm and the Schur kernel's fill-in tasks (not
+  // when the trsm must wait for other work, nor with FP64 pivots for FP32
+  // blocks or BF16 Schur products, which the separate kernels implement)
+  const bool fused = NF == 32 && before_trsm == nullptr && !bf16 &&
+                     !((prec & kPrecF64Pivots) != 0 && sizeof(T) == 4);
+  if (fused) {
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (kt_kernel_events(kKcPotrf, &ea, &eb))
+      hipExtLaunchKernelGGL(k_node32<T>, dim3(lv.dim, B), dim3(64), node32_lds(sizeof(T)), st, ea, eb, 0, L, E, mask, N,
+                            nE, lv.s, lv.off, lv.next_off, Lsrc, Esrc);
+    else
+      hipLaunchKernelGGL(k_node32<T>, dim3(lv.dim, B), dim3(64), node32_lds(sizeof(T)), st, L, E, mask, N, nE, lv.s,
+                         lv.off, lv.next_off, Lsrc, Esrc);
+    hipError_t e = hipSuccess;
+    if (after_potrf != nullptr) e = hipEventRecord(after_potrf, st);
+    if (e == hipSuccess && after_trsm != nullptr) e = hipEventRecord(after_trsm, st);
+    if (e != hipSuccess) return e;
+    if (lidx == Lv - 1) return hipGetLastError();
+  } else {
     // under socu_profile_op the launch itself records its start and end
     // (events of hipExtLaunchKernel: the kernel's own execution interval)
     auto k = k_potrf_blk<T, NB, NF, T>;
@@ -1461,38 +1588,39 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
                             Lsrc);
     else
       hipLaunchKernelGGL(k, dim3(lv.dim, B), dim3(potrf_threads(NB)), lds_potrf, st, L, mask, N, n, lv.s, Lsrc);
-  }
-  if (after_potrf != nullptr) {
-    const hipError_t e = hipEventRecord(after_potrf, st);
-    if (e != hipSuccess) return e;
-  }
-  if (lidx == Lv - 1) return hipGetLastError();
-  if (before_trsm != nullptr) {
-    const hipError_t e = hipStreamWaitEvent(st, before_trsm, 0);
-    if (e != hipSuccess) return e;
-  }
-  kt_begin(kKcTrsm, st);
-  if (B * lv.dim >= trsm_node_min(NB)) {
-    const int cpb = trsm_chunks_per_block();
-    const int groups = (2 * NB + cpb - 1) / cpb;
-    const size_t lds_node = trsm_node_lds<T, NB>();
-    hipLaunchKernelGGL((k_trsm_node<T, NB, NF>), dim3(lv.dim, groups, B), dim3(kTrsmThreads), lds_node, st, L, E, mask,
-                       N, n, nE, lv.s, lv.off, cpb, Esrc);
-  }
-  else
-    hipLaunchKernelGGL((k_trsm_blk<T, NB, NF>), dim3(lv.dim, 2 * NB, B), dim3(trsm_blk_threads(NB)), lds_trsm, st, L, E,
-                       mask, N, n, nE, lv.s, lv.off, Esrc);
-  kt_end(kKcTrsm, st);
-  if (after_trsm != nullptr) {
-    const hipError_t e = hipEventRecord(after_trsm, st);
-    if (e != hipSuccess) return e;
+    if (after_potrf != nullptr) {
+      const hipError_t e = hipEventRecord(after_potrf, st);
+      if (e != hipSuccess) return e;
+    }
+    if (lidx == Lv - 1) return hipGetLastError();
+    if (before_trsm != nullptr) {
+      const hipError_t e = hipStreamWaitEvent(st, before_trsm, 0);
+      if (e != hipSuccess) return e;
+    }
+    kt_begin(kKcTrsm, st);
+    if (B * lv.dim >= trsm_node_min(NB)) {
+      const int cpb = trsm_chunks_per_block();
+      const int groups = (2 * NB + cpb - 1) / cpb;
+      const size_t lds_node = trsm_node_lds<T, NB>();
+      hipLaunchKernelGGL((k_trsm_node<T, NB, NF>), dim3(lv.dim, groups, B), dim3(kTrsmThreads), lds_node, st, L, E,
+                         mask, N, n, nE, lv.s, lv.off, cpb, Esrc);
+    }
+    else
+      hipLaunchKernelGGL((k_trsm_blk<T, NB, NF>), dim3(lv.dim, 2 * NB, B), dim3(trsm_blk_threads(NB)), lds_trsm, st, L,
+                         E, mask, N, n, nE, lv.s, lv.off, Esrc);
+    kt_end(kKcTrsm, st);
+    if (after_trsm != nullptr) {
+      const hipError_t e = hipEventRecord(after_trsm, st);
+      if (e != hipSuccess) return e;
+    }
   }
   const int ntile = (n + kSTile - 1) / kSTile;
   const int ntri = ntile * (ntile + 1) / 2;
   const int nT = N / (2 * lv.s);
-  int nF = 0;
-  for (int t = 1; t < lv.dim; ++t)
-    if (lv.s - 1 + 2 * lv.s * t + lv.s < N) ++nF;
+  int nF = 0;  // fill-in tasks (k_node32 forms the fill-in blocks itself)
+  if (!fused)
+    for (int t = 1; t < lv.dim; ++t)
+      if (lv.s - 1 + 2 * lv.s * t + lv.s < N) ++nF;
   const int tasks = nT * ntri + nF * ntile * ntile;
   if (tasks > 0) {
     kt_begin(kKcSchur, st);

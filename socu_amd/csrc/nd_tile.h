@@ -233,8 +233,14 @@ __device__ __forceinline__ void acc16_foreach(typename Mfma<T>::acc_t& acc, F f)
 // lanes only change entries of their strict upper triangle, which nothing
 // reads (not the column buffers, the pivots, nor the store), and their own
 // pivot entries are replaced explicitly.
-template <typename T>
-__device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col, T* ut) {
+// hook(k) runs at the start of pair step k (k even, a compile-time constant
+// after unrolling): a caller can issue loads there, late in the chain, when
+// the column registers of the early steps are free.
+struct NoHook {
+  __device__ __forceinline__ void operator()(int) const {}
+};
+template <typename T, typename Hook = NoHook>
+__device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col, T* ut, Hook hook = Hook()) {
   const int lane = lane_id();
   const int r = lane & 31;
   const bool rowlane = lane < 32;
@@ -305,6 +311,7 @@ __device__ __forceinline__ void tile_chol_inv(T* S, int pitch, T* dinv, T* col, 
         c1[q] = cb[32 + q];
       }
     }
+    hook(k);
     const T a = bcast_lane(u[k], k);
     const T b = bcast_lane(u[k], k + 1);
     const T c = bcast_lane(u[k + 1], k + 1);
