@@ -158,6 +158,11 @@ int run_op(int op, void* L, void* E, void* x, const uint8_t* mask, int B, int N,
     case SOCU_OP_SOLVE:
       return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, true, true, sw, st);
     case SOCU_OP_FACTOR_AND_SOLVE:
+      if (socu::fused_forward_applies<T>(n, 0)) {  // forward sweep inside the factor's kernels
+        const hipError_t e = socu::factor_and_forward<T>(Lt, Et, xt, mask, B, N, n, m, 0, st, Ls, Es);
+        if (e != hipSuccess) return hip_fail(e, "factor_and_solve launch");
+        return run_sweeps<T>(Lt, Et, xt, mask, B, N, n, m, false, true, sw, st);
+      }
       if (ov != nullptr) {  // forward sweep alongside the factor, then the backward sweep
         const hipError_t e =
             socu::factor_with_forward<T>(Lt, Et, xt, mask, mask, B, N, n, m, false, *ov, st, nullptr, Ls, Es);

@@ -296,13 +296,20 @@ __global__ __launch_bounds__(kThreads) void k_potrf_blk(TS* __restrict__ L, cons
 // R_i = block (i+s, i) exists when i + s < N, Lc_i = block (i, i-s) when
 // i >= s, F when both do.  Layouts and storage as the separate kernels;
 // Lsrc / Esrc (optional) are the out-of-place sources of L_i, R_i and Lc_i.
+// With x (n x m blocks, optional) the node also takes its forward-sweep step
+// y_i <- Linv y_i (k_apply_fwd); the updates y_j -= ... that level l sends to
+// its targets are then k_schur_small's, so that a factor_and_solve needs no
+// separate forward-sweep kernels.
 // ---------------------------------------------------------------------------
 constexpr size_t node32_lds(size_t esize) { return esize * (2 * 32 * 33 + 32 + 128); }
 
+// (two wavefronts per SIMD: the register budget is held to 256, which the
+// compiler meets without spills by keeping the MFMA accumulators in VGPRs)
 template <typename T>
-__global__ __launch_bounds__(64) void k_node32(T* __restrict__ L, T* __restrict__ E, const uint8_t* __restrict__ mask,
+__global__ __launch_bounds__(64, 2) void k_node32(T* __restrict__ L, T* __restrict__ E, const uint8_t* __restrict__ mask,
                                               int N, int nE, int s, int off, int next_off,
-                                              const T* __restrict__ Lsrc, const T* __restrict__ Esrc) {
+                                              const T* __restrict__ Lsrc, const T* __restrict__ Esrc,
+                                              T* __restrict__ x, int m) {
   constexpr int n = 32, P = 33;
   constexpr size_t nn = 32 * 32;
   const int b = blockIdx.y;
@@ -339,8 +346,17 @@ __global__ __launch_bounds__(64) void k_node32(T* __restrict__ L, T* __restrict_
   T rv[16], cv[16];
   const T* XRin = hasR ? ((Esrc != nullptr) ? Esrc + (XR - E) : XR) : nullptr;
   const T* XCin = hasC ? ((Esrc != nullptr) ? Esrc + (XC - E) : XC) : nullptr;
+  // y_i (forward sweep): up to 4 columns (128 entries, two per lane) are
+  // loaded with R_i and Lc_i; wider ones at their step
+  T* y = (x != nullptr) ? x + ((size_t)b * N + i) * (size_t)n * m : nullptr;
+  const bool ypre = y != nullptr && m <= 4;
+  T yv0 = T(0), yv1 = T(0);
   auto hook = [&](int k) {
     if (k == 20) {
+      if (ypre) {
+        yv0 = y[min(l, 32 * m - 1)];
+        yv1 = y[min(l + 64, 32 * m - 1)];
+      }
       if (hasR) {
 #pragma unroll
         for (int q = 0; q < 16; ++q) rv[q] = XRin[(2 * q + lr) * n + lc];
@@ -355,6 +371,29 @@ __global__ __launch_bounds__(64) void k_node32(T* __restrict__ L, T* __restrict_
   // the factor (lower) with the transposed inverse (strict upper)
 #pragma unroll
   for (int q = 0; q < 16; ++q) A[(2 * q + lr) * n + lc] = S[(2 * q + lr) * P + lc];
+  if (y != nullptr) {
+    // y_i <- Linv y_i, up to 32 columns at a time staged in S:
+    // out[r][c] = sum_k ut[k][r] y[k][c]; lane l forms row l % 32 of the
+    // columns of its half (l / 32)
+    for (int c0 = 0; c0 < m; c0 += 32) {
+      const int w = min(32, m - c0);
+      if (ypre) {  // entries l and l + 64 of the 32 x m block
+        if (l < 32 * m) S[(l / m) * P + l % m] = yv0;
+        if (l + 64 < 32 * m) S[((l + 64) / m) * P + (l + 64) % m] = yv1;
+      } else {
+        for (int q = l; q < 32 * w; q += 64) {
+          const int k = q / w, c = q - k * w;
+          S[k * P + c] = y[(size_t)k * m + c0 + c];
+        }
+      }
+      for (int c = lr; c < w; c += 2) {
+        T acc = T(0);
+#pragma unroll
+        for (int k = 0; k < 32; ++k) acc = fma(ut[k * P + lc], S[k * P + c], acc);
+        y[(size_t)lc * m + c0 + c] = acc;
+      }
+    }
+  }
   if (!hasR && !hasC) return;
 
   // (LDS operations of one wavefront complete in issue order: S and ut are
@@ -985,9 +1024,13 @@ __global__ __launch_bounds__(kThreads) void k_schur_blk(T* __restrict__ L, T* __
 // the MFMA products.  Panels are kept as P[i][k] (pitch NP + 1): row panels
 // are 32 rows of a block, column panels 32 of its columns, transposed.
 // ---------------------------------------------------------------------------
+// four panels of 32 x (NP+1), then (forward sweep in the same launch) the
+// two neighbours' right-hand sides, NP x kSmallYc each, and 32 rows of the
+// target's, 32 x kSmallYc
+constexpr int kSmallYc = 8;
 template <typename T, int NB>
 constexpr size_t schur_small_lds() {
-  return sizeof(T) * 4 * 32 * (size_t)(32 * NB + 1);
+  return sizeof(T) * (4 * 32 * (size_t)(32 * NB + 1) + 2 * (size_t)(32 * NB) * kSmallYc + 32 * kSmallYc);
 }
 
 template <typename T, int NB>
@@ -1029,10 +1072,18 @@ __device__ __forceinline__ void panel_store(T* P, const T* v, bool cols, int i0,
   }
 }
 
+// With x (optional; n x m blocks of right-hand sides) the diagonal-tile
+// tasks of a target j also apply level l's forward-sweep update to rows
+// 32 tr .. of y_j (k_sweep_update<fwd>):
+//   y_j -= R_{j-s} y_{j-s} + Lc_{j+s}^T y_{j+s},
+// with the operand panels the Schur product has staged: rows 32 tr.. of R and
+// columns 32 tr.. of Lc.  The nodes' y must have had their level-l step
+// y <- Linv y (k_node32 does it).
 template <typename T, int NB, int NF, bool BF = false>
 __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* __restrict__ E,
                                                          const uint8_t* __restrict__ mask, int N, int n_, int nE, int s,
-                                                         int off, int next_off, int nT, const T* __restrict__ Lsrc) {
+                                                         int off, int next_off, int nT, const T* __restrict__ Lsrc,
+                                                         T* __restrict__ x, int m) {
   const int n = NF ? NF : n_;
   constexpr int NP = 32 * NB, PP = NP + 1;
   constexpr int PE = 32 * NP / kThreads;
@@ -1051,6 +1102,7 @@ __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* 
   T* C;
   int tr, tc;
   bool diag;
+  int jt = -1;  // the target (target tasks only)
   if (task < nT * NTRI) {
     // L_j -= R R^T + Lc^T Lc  (R = block (j, j-s), Lc = block (j+s, j))
     const int u = task / NTRI;
@@ -1059,6 +1111,7 @@ __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* 
     while (q > tr) { q -= tr + 1; ++tr; }
     tc = q;
     const int j = 2 * s * (u + 1) - 1;
+    jt = j;
     C = L + ((size_t)b * N + j) * nn;
     const T* R = Eb + (size_t)(off + (j - s) / s) * nn;
     const bool hasC = j + s < N;
@@ -1087,7 +1140,34 @@ __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* 
   }
   const int l = lane_id(), wave = wave_id();
   const int wi = wave >> 1, wj = wave & 1;
-  // every load of the task at once: panels, then this wave's C values
+  // forward sweep (diagonal tiles of targets, with x): rows 32 tr .. of y_j
+  // -= R_{j-s} y_{j-s} + Lc_{j+s}^T y_{j+s}, kSmallYc columns at a time
+  const bool ysw = x != nullptr && diag;
+  const size_t nm = (size_t)n * m;
+  const T* ya = nullptr;
+  const T* yc = nullptr;
+  T* yj = nullptr;
+  if (ysw) {
+    T* xb = x + (size_t)b * N * nm;
+    ya = xb + (size_t)(jt - s) * nm;
+    yc = (np == 2) ? xb + (size_t)(jt + s) * nm : ya;
+    yj = xb + (size_t)jt * nm;
+  }
+  constexpr int YA = 32 * NB * kSmallYc / kThreads;  // neighbour entries per thread
+  T yav[YA], ycv[YA], yjv = T(0);
+  // (unconditional loads from clamped addresses; masked when staged)
+  auto yload = [&](int c0, int w) {
+#pragma unroll
+    for (int e = 0; e < YA; ++e) {
+      const int q = threadIdx.x + e * kThreads, k = q / w, c = q - k * w;
+      yav[e] = ya[(size_t)min(k, n - 1) * m + c0 + c];
+      ycv[e] = yc[(size_t)min(k, n - 1) * m + c0 + c];
+    }
+    const int q = threadIdx.x, i = q / w, c = q - i * w;
+    yjv = yj[(size_t)min(32 * tr + i, n - 1) * m + c0 + c];
+  };
+  // every load of the task at once: panels, then this wave's C values, then
+  // the first columns of the right-hand sides
   T v[4][PE];
 #pragma unroll
   for (int p = 0; p < 4; ++p)
@@ -1108,6 +1188,7 @@ __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* 
   for (int p = 0; p < 4; ++p)
 #pragma unroll
     for (int e = 0; e < PE; ++e) pin_reg(v[p][e]);
+  if (ysw) yload(0, min(kSmallYc, m));
 #pragma unroll
   for (int r = 0; r < 4; ++r) pin_reg(cv[r]);
 #pragma unroll
@@ -1122,6 +1203,52 @@ __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* 
   for (int r = 0; r < 4; ++r) {
     const int rr = 32 * tr + 16 * wi + Mfma<T>::row(l, r), cc = 32 * tc + 16 * wj + Mfma<T>::col(l, r);
     if (rr < n && cc < n && (!diag || cc <= rr)) C[rr * n + cc] = cv[r] - acc[r];
+  }
+  if (!ysw) return;
+  const bool hasC = np == 2;
+  T* Ya = Pn + 4 * 32 * PP;  // NP x kSmallYc
+  T* Yc = Ya + NP * kSmallYc;
+  T* Yj = Yc + NP * kSmallYc;  // 32 x kSmallYc
+  const T* P0 = Pn;               // P0[i][k] = R[32 tr + i][k]
+  const T* P2 = Pn + 2 * 32 * PP;  // P2[i][k] = Lc[k][32 tr + i]
+  constexpr int KP = NP / 8;
+  const int tid = threadIdx.x, part = tid & 7;
+  for (int c0 = 0; c0 < m; c0 += kSmallYc) {
+    const int w = min(kSmallYc, m - c0);
+    if (c0 > 0) yload(c0, w);
+#pragma unroll
+    for (int e = 0; e < YA; ++e) {
+      const int q = tid + e * kThreads, k = q / w, c = q - k * w;
+      if (q < NP * w) {
+        Ya[k * kSmallYc + c] = (k < n) ? yav[e] : T(0);
+        Yc[k * kSmallYc + c] = (hasC && k < n) ? ycv[e] : T(0);
+      }
+    }
+    if (tid < 32 * w) Yj[(tid / w) * kSmallYc + tid % w] = yjv;
+    __syncthreads();
+    // 8 lanes per output, each summing NP/8 of K, then a width-8 shuffle
+    // reduction
+    for (int q = tid >> 3; q < 32 * w; q += kThreads / 8) {
+      const int i = q & 31, c = q >> 5;
+      T acc = T(0);
+#pragma unroll
+      for (int e = 0; e < KP; ++e) {
+        const int k = part * KP + e;
+        acc += P0[i * PP + k] * Ya[k * kSmallYc + c];
+      }
+      if (hasC) {
+#pragma unroll
+        for (int e = 0; e < KP; ++e) {
+          const int k = part * KP + e;
+          acc += P2[i * PP + k] * Yc[k * kSmallYc + c];
+        }
+      }
+#pragma unroll
+      for (int o = 4; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 8);
+      const int r = 32 * tr + i;
+      if (part == 0 && r < n) yj[(size_t)r * m + c0 + c] = Yj[i * kSmallYc + c] - acc;
+    }
+    __syncthreads();
   }
 }
 
@@ -1547,7 +1674,8 @@ hipError_t init_blocked_kernels() {
 template <typename T, int NB, int NF>
 static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N, int n, int lidx, int prec,
                                   hipStream_t st, hipEvent_t after_potrf, hipEvent_t after_trsm,
-                                  hipEvent_t before_trsm, const T* Lsrc, const T* Esrc) {
+                                  hipEvent_t before_trsm, const T* Lsrc, const T* Esrc, T* x = nullptr,
+                                  int m = 0) {
   const int nE = (int)off_diag_storage_len(N);
   const int Lv = recursive_iterations(N);
   const Level lv = level_info(N, lidx);
@@ -1558,16 +1686,20 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
   // level instead of potrf, trsm and the Schur kernel's fill-in tasks (not
   // when the trsm must wait for other work, nor with FP64 pivots for FP32
   // blocks or BF16 Schur products, which the separate kernels implement)
+  // With x (fused_forward_applies), the level's forward-sweep steps run in
+  // the same kernels: y_i <- Linv y_i in k_node32, the updates of the targets'
+  // y in k_schur_small.
   const bool fused = NF == 32 && before_trsm == nullptr && !bf16 &&
                      !((prec & kPrecF64Pivots) != 0 && sizeof(T) == 4);
+  T* const xf = fused ? x : nullptr;
   if (fused) {
     hipEvent_t ea = nullptr, eb = nullptr;
     if (kt_kernel_events(kKcPotrf, &ea, &eb))
       hipExtLaunchKernelGGL(k_node32<T>, dim3(lv.dim, B), dim3(64), node32_lds(sizeof(T)), st, ea, eb, 0, L, E, mask, N,
-                            nE, lv.s, lv.off, lv.next_off, Lsrc, Esrc);
+                            nE, lv.s, lv.off, lv.next_off, Lsrc, Esrc, xf, m);
     else
       hipLaunchKernelGGL(k_node32<T>, dim3(lv.dim, B), dim3(64), node32_lds(sizeof(T)), st, L, E, mask, N, nE, lv.s,
-                         lv.off, lv.next_off, Lsrc, Esrc);
+                         lv.off, lv.next_off, Lsrc, Esrc, xf, m);
     hipError_t e = hipSuccess;
     if (after_potrf != nullptr) e = hipEventRecord(after_potrf, st);
     if (e == hipSuccess && after_trsm != nullptr) e = hipEventRecord(after_trsm, st);
@@ -1624,14 +1756,14 @@ static hipError_t factor_level_nb(T* L, T* E, const uint8_t* mask, int B, int N,
   const int tasks = nT * ntri + nF * ntile * ntile;
   if (tasks > 0) {
     kt_begin(kKcSchur, st);
-    if (use_schur_small(B * (nT + nF), n, (int)sizeof(T))) {
+    if (xf != nullptr || use_schur_small(B * (nT + nF), n, (int)sizeof(T))) {
       const int nsmall = nT * (NB * (NB + 1) / 2) + nF * NB * NB;
       const size_t lds_small = schur_small_lds<T, NB>();
       auto k = k_schur_small<T, NB, NF, false>;
       if constexpr (sizeof(T) == 4)
         if (bf16) k = k_schur_small<T, NB, NF, true>;
       hipLaunchKernelGGL(k, dim3(nsmall, B), dim3(kThreads), lds_small, st, L, E, mask, N, n, nE, lv.s, lv.off,
-                         lv.next_off, nT, Lsrc);
+                         lv.next_off, nT, Lsrc, xf, m);
     } else {
       auto k = k_schur_blk<T, NF, false>;
       if constexpr (sizeof(T) == 4)
@@ -1892,6 +2024,23 @@ void overlap_destroy(SweepOverlap& ov) {
   ov = SweepOverlap();
 }
 
+template <typename T>
+bool fused_forward_applies(int n, int prec) {
+  return n == 32 && !(sizeof(T) == 4 && (prec & (kPrecF64Pivots | kPrecBf16Schur)) != 0);
+}
+
+template <typename T>
+hipError_t factor_and_forward(T* L, T* E, T* x, const uint8_t* mask, int B, int N, int n, int m, int prec,
+                              hipStream_t st, const T* Lsrc, const T* Esrc) {
+  if (!fused_forward_applies<T>(n, prec)) return hipErrorInvalidValue;
+  const int Lv = recursive_iterations(N);
+  hipError_t e = hipSuccess;
+  for (int l = 0; l < Lv && e == hipSuccess; ++l)
+    e = factor_level_nb<T, 1, 32>(L, E, mask, B, N, n, l, prec, st, nullptr, nullptr, nullptr, l == 0 ? Lsrc : nullptr,
+                                  l == 0 ? Esrc : nullptr, x, m);
+  return e;
+}
+
 // The reference's fused factor_and_solve likewise interleaves the forward
 // sweep into the factor's levels (block_tridiag_solver.py:926-1233).  Here
 // the sweep runs on a second stream: the inverses of level l follow that
@@ -1903,6 +2052,9 @@ hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, con
                                int n, int m, int prec, const SweepOverlap& ov, hipStream_t st,
                                const std::function<hipError_t(hipStream_t)>* side_first, const T* Lsrc,
                                const T* Esrc) {
+  // n == 32: the forward sweep runs inside the factor's own kernels
+  if (side_first == nullptr && factor_mask == sweep_mask && fused_forward_applies<T>(n, prec))
+    return factor_and_forward<T>(L, E, x, factor_mask, B, N, n, m, prec, st, Lsrc, Esrc);
   const int Lv = recursive_iterations(N);
   hipError_t e = hipEventRecord(ov.fork, st);
   if (e == hipSuccess) e = hipStreamWaitEvent(ov.side, ov.fork, 0);
@@ -1923,6 +2075,12 @@ hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, con
   return e;
 }
 
+template bool fused_forward_applies<float>(int, int);
+template bool fused_forward_applies<double>(int, int);
+template hipError_t factor_and_forward<float>(float*, float*, float*, const uint8_t*, int, int, int, int, int,
+                                              hipStream_t, const float*, const float*);
+template hipError_t factor_and_forward<double>(double*, double*, double*, const uint8_t*, int, int, int, int, int,
+                                               hipStream_t, const double*, const double*);
 template hipError_t factor_with_forward<float>(float*, float*, float*, const uint8_t*, const uint8_t*, int, int, int,
                                                int, int, const SweepOverlap&, hipStream_t,
                                                const std::function<hipError_t(hipStream_t)>*, const float*,

@@ -164,6 +164,18 @@ hipError_t factor_with_forward(T* L, T* E, T* x, const uint8_t* factor_mask, con
                                const std::function<hipError_t(hipStream_t)>* side_first = nullptr,
                                const T* Lsrc = nullptr, const T* Esrc = nullptr);
 
+// n == 32 (blocks in the storage precision): the factor levels with the
+// forward sweep of x inside the same kernels (k_node32, k_schur_small), one
+// stream, no separate sweep kernels; x then holds y for the backward sweep.
+// fused_forward_applies says whether (n, prec) takes this path;
+// factor_and_forward returns hipErrorInvalidValue otherwise.  Lsrc / Esrc as
+// for factor_level_blocked.
+template <typename T>
+bool fused_forward_applies(int n, int prec);
+template <typename T>
+hipError_t factor_and_forward(T* L, T* E, T* x, const uint8_t* mask, int B, int N, int n, int m, int prec,
+                              hipStream_t st, const T* Lsrc = nullptr, const T* Esrc = nullptr);
+
 // Mixed-precision factor_and_solve with FP64 refinement (nd_refine.hip).
 struct MixedPlan {
   const double* L = nullptr;  // Psi, FP64, untouched

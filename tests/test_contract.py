@@ -301,7 +301,8 @@ def _capi_codes(dtype):
 
 
 @pytest.mark.parametrize("n,N,m,dtype", [(128, 24, 1, "float64"), (40, 33, 2, "float64"), (64, 50, 1, "float32"),
-                                         (32, 1, 1, "float64"), (8, 2, 3, "float32"), (160, 9, 2, "float64")])
+                                         (32, 1, 1, "float64"), (8, 2, 3, "float32"), (160, 9, 2, "float64"),
+                                         (32, 45, 40, "float32")])
 def test_out_of_place_factor_equals_in_place(n, N, m, dtype, device):
     """socu_cholesky_factor_from / _and_solve_from: bit-identical to the
     in-place entry points on copies of the inputs (the same kernels read the
