@@ -1024,13 +1024,14 @@ __global__ __launch_bounds__(kThreads) void k_schur_blk(T* __restrict__ L, T* __
 // the MFMA products.  Panels are kept as P[i][k] (pitch NP + 1): row panels
 // are 32 rows of a block, column panels 32 of its columns, transposed.
 // ---------------------------------------------------------------------------
-// four panels of 32 x (NP+1), then (forward sweep in the same launch) the
-// two neighbours' right-hand sides, NP x kSmallYc each, and 32 rows of the
-// target's, 32 x kSmallYc
+// four panels of 32 x (NP+1).  (The forward sweep of diagonal tiles stages
+// the two neighbours' right-hand sides, NP x kSmallYc each, and 32 rows of the
+// target's, 32 x kSmallYc, in panel 1, which diagonal tiles do not use.)
 constexpr int kSmallYc = 8;
+static_assert(2 * 32 * kSmallYc + 32 * kSmallYc <= 32 * 33, "right-hand sides fit in one panel");
 template <typename T, int NB>
 constexpr size_t schur_small_lds() {
-  return sizeof(T) * (4 * 32 * (size_t)(32 * NB + 1) + 2 * (size_t)(32 * NB) * kSmallYc + 32 * kSmallYc);
+  return sizeof(T) * 4 * 32 * (size_t)(32 * NB + 1);
 }
 
 template <typename T, int NB>
@@ -1166,12 +1167,14 @@ __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* 
     const int q = threadIdx.x, i = q / w, c = q - i * w;
     yjv = yj[(size_t)min(32 * tr + i, n - 1) * m + c0 + c];
   };
-  // every load of the task at once: panels, then this wave's C values, then
-  // the first columns of the right-hand sides
+  // every load of the task at once: the first columns of the right-hand
+  // sides, the panels, then this wave's C values.  (Diagonal tiles: panels 1
+  // and 3 equal panels 0 and 2 and are neither loaded nor stored.)
+  if (ysw) yload(0, min(kSmallYc, m));
   T v[4][PE];
 #pragma unroll
   for (int p = 0; p < 4; ++p)
-    if (p < 2 * np) panel_load<T, NB>(v[p], M[p], cols[p], i0[p], n);
+    if (p < 2 * np && !(diag && (p & 1))) panel_load<T, NB>(v[p], M[p], cols[p], i0[p], n);
   // C values: the target's (the fill-in block receives its only update here,
   // F = -R Lc, so its entry contents are not read)
   T cv[4] = {T(0), T(0), T(0), T(0)};
@@ -1188,67 +1191,69 @@ __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* 
   for (int p = 0; p < 4; ++p)
 #pragma unroll
     for (int e = 0; e < PE; ++e) pin_reg(v[p][e]);
-  if (ysw) yload(0, min(kSmallYc, m));
 #pragma unroll
   for (int r = 0; r < 4; ++r) pin_reg(cv[r]);
 #pragma unroll
   for (int p = 0; p < 4; ++p)
-    if (p < 2 * np) panel_store<T, NB>(Pn + p * 32 * PP, v[p], cols[p], i0[p], n);
+    if (p < 2 * np && !(diag && (p & 1))) panel_store<T, NB>(Pn + p * 32 * PP, v[p], cols[p], i0[p], n);
+  const bool hasC = np == 2;
+  T* Ya = Pn + 32 * PP;  // (panel 1) NP x kSmallYc
+  T* Yc = Ya + NP * kSmallYc;
+  T* Yj = Yc + NP * kSmallYc;  // 32 x kSmallYc
+  auto ystage = [&](int w) {
+#pragma unroll
+    for (int e = 0; e < YA; ++e) {
+      const int q = threadIdx.x + e * kThreads, k = q / w, c = q - k * w;
+      if (q < NP * w) {
+        Ya[k * kSmallYc + c] = (k < n) ? yav[e] : T(0);
+        Yc[k * kSmallYc + c] = (hasC && k < n) ? ycv[e] : T(0);
+      }
+    }
+    if ((int)threadIdx.x < 32 * w) Yj[(threadIdx.x / w) * kSmallYc + threadIdx.x % w] = yjv;
+  };
+  if (ysw) ystage(min(kSmallYc, m));  // published by the panels' barrier
   __syncthreads();
   typename Mfma<T>::acc_t acc = acc_zero<T>();
   for (int p = 0; p < np; ++p)
-    schur_gemm16<T, BF>(acc, Pn + (2 * p) * 32 * PP + 16 * wi * PP, PP, 1, Pn + (2 * p + 1) * 32 * PP + 16 * wj * PP,
-                        1, PP, NP);
+    schur_gemm16<T, BF>(acc, Pn + (2 * p) * 32 * PP + 16 * wi * PP, PP, 1,
+                        Pn + (diag ? 2 * p : 2 * p + 1) * 32 * PP + 16 * wj * PP, 1, PP, NP);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int rr = 32 * tr + 16 * wi + Mfma<T>::row(l, r), cc = 32 * tc + 16 * wj + Mfma<T>::col(l, r);
     if (rr < n && cc < n && (!diag || cc <= rr)) C[rr * n + cc] = cv[r] - acc[r];
   }
   if (!ysw) return;
-  const bool hasC = np == 2;
-  T* Ya = Pn + 4 * 32 * PP;  // NP x kSmallYc
-  T* Yc = Ya + NP * kSmallYc;
-  T* Yj = Yc + NP * kSmallYc;  // 32 x kSmallYc
   const T* P0 = Pn;               // P0[i][k] = R[32 tr + i][k]
   const T* P2 = Pn + 2 * 32 * PP;  // P2[i][k] = Lc[k][32 tr + i]
-  constexpr int KP = NP / 8;
-  const int tid = threadIdx.x, part = tid & 7;
+  const int tid = threadIdx.x;
   for (int c0 = 0; c0 < m; c0 += kSmallYc) {
     const int w = min(kSmallYc, m - c0);
-    if (c0 > 0) yload(c0, w);
-#pragma unroll
-    for (int e = 0; e < YA; ++e) {
-      const int q = tid + e * kThreads, k = q / w, c = q - k * w;
-      if (q < NP * w) {
-        Ya[k * kSmallYc + c] = (k < n) ? yav[e] : T(0);
-        Yc[k * kSmallYc + c] = (hasC && k < n) ? ycv[e] : T(0);
-      }
+    if (c0 > 0) {
+      __syncthreads();  // every wave is done with the previous columns
+      yload(c0, w);
+      ystage(w);
+      __syncthreads();
     }
-    if (tid < 32 * w) Yj[(tid / w) * kSmallYc + tid % w] = yjv;
-    __syncthreads();
-    // 8 lanes per output, each summing NP/8 of K, then a width-8 shuffle
-    // reduction
-    for (int q = tid >> 3; q < 32 * w; q += kThreads / 8) {
-      const int i = q & 31, c = q >> 5;
-      T acc = T(0);
-#pragma unroll
-      for (int e = 0; e < KP; ++e) {
-        const int k = part * KP + e;
-        acc += P0[i * PP + k] * Ya[k * kSmallYc + c];
+    // one output per thread, (i, c) = (tid / w, tid % w): consecutive
+    // threads store consecutive entries of y_j's rows
+    if (tid < 32 * w) {
+      const int i = tid / w, c = tid - i * w;
+      T a0 = T(0), a1 = T(0);
+#pragma unroll 8
+      for (int k = 0; k < NP; k += 2) {
+        a0 = fma(P0[i * PP + k], Ya[k * kSmallYc + c], a0);
+        a1 = fma(P0[i * PP + k + 1], Ya[(k + 1) * kSmallYc + c], a1);
       }
       if (hasC) {
-#pragma unroll
-        for (int e = 0; e < KP; ++e) {
-          const int k = part * KP + e;
-          acc += P2[i * PP + k] * Yc[k * kSmallYc + c];
+#pragma unroll 8
+        for (int k = 0; k < NP; k += 2) {
+          a0 = fma(P2[i * PP + k], Yc[k * kSmallYc + c], a0);
+          a1 = fma(P2[i * PP + k + 1], Yc[(k + 1) * kSmallYc + c], a1);
         }
       }
-#pragma unroll
-      for (int o = 4; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 8);
       const int r = 32 * tr + i;
-      if (part == 0 && r < n) yj[(size_t)r * m + c0 + c] = Yj[i * kSmallYc + c] - acc;
+      if (r < n) yj[(size_t)r * m + c0 + c] = Yj[i * kSmallYc + c] - (a0 + a1);
     }
-    __syncthreads();
   }
 }
 
