@@ -1,16 +1,18 @@
-"""n = 32 factor_and_solve against the oracle.
+"""factor_and_solve against the oracle, with the n = 32 fused kernels.
 
 At n = 32 one kernel per level factors each node's diagonal block, applies
 the coupling trsm and forms the fill-in block (the reference's potrf, trsm
 and gemm passes, socu/blas/potrf.py:97-119, socu/blas/trsm.py:94-120 and
 :202-228, block_tridiag_solver.py:797-813), and a factor_and_solve takes the
 forward sweep inside the factor's kernels (block_tridiag_solver.py:926-1233
-interleaves them likewise).  Checked: tril(L) of every block, every E index
-(fill-in blocks included, and the same blocks zero as the oracle's) and x,
-FP64 to 1e-11 and FP32 to 1e-4 relative (the oracle in FP64); masked
-systems untouched.  Right-hand-side widths cross the kernels' column chunks
-(8 in the Schur kernel, 32 in the node kernel); both the stateless C entry
-point and the launch object (captured in a graph) are used.
+interleaves them likewise); other n run the level kernels with the forward
+sweep beside the factor.  Checked for several n: tril(L) of every block,
+every E index (fill-in blocks included, and the same blocks zero as the
+oracle's) and x, FP64 to 1e-11 and FP32 to 5e-4 relative (the oracle in
+FP64); masked systems untouched.  Right-hand-side widths cross the kernels'
+column chunks (8 in the Schur kernel, 32 in the node kernel, 16 in the sweep
+kernels); both the stateless C entry point and the launch object (captured
+in a graph) are used.
 """
 import numpy as np
 import pytest
@@ -19,7 +21,7 @@ from oracle import nd
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"float64": 1e-11, "float32": 1e-4}
+TOL = {"float64": 1e-11, "float32": 5e-4}
 
 
 def _rel(a, b):
@@ -49,9 +51,14 @@ def _check(Lg, Eg, xg, L0, E0, b0, mask, dtype):
         assert _rel(xg[k].astype(np.float64), xr[k]) < tol
 
 
+CASES = [(32, 1, 3), (32, 2, 1), (32, 3, 9), (32, 33, 40), (32, 64, 1), (32, 100, 9), (32, 17, 33),
+         (8, 20, 3), (20, 37, 1), (40, 33, 2), (64, 64, 1), (64, 21, 17), (96, 17, 3), (100, 12, 9), (128, 40, 1),
+         (128, 19, 10), (128, 2, 1)]
+
+
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
-@pytest.mark.parametrize("N,m", [(1, 3), (2, 1), (3, 9), (33, 40), (64, 1), (100, 9), (17, 33)])
-def test_factor_and_solve_entry_point(N, m, dtype, device):
+@pytest.mark.parametrize("n,N,m", CASES)
+def test_factor_and_solve_entry_point(n, N, m, dtype, device):
     import torch
 
     from socu_amd import _lib
@@ -59,8 +66,8 @@ def test_factor_and_solve_entry_point(N, m, dtype, device):
     lib = _lib.load()
     code = _lib.DTYPE_F64 if dtype == "float64" else _lib.DTYPE_F32
     tdt = getattr(torch, dtype)
-    n, B = 32, 3
-    L0, E0, b0 = _stack(n, N, m, B, seed=11 * N + m)
+    B = 3
+    L0, E0, b0 = _stack(n, N, m, B, seed=11 * N + m + n)
     L = torch.tensor(L0, dtype=tdt, device=device)
     E = torch.tensor(E0, dtype=tdt, device=device)
     x = torch.tensor(b0, dtype=tdt, device=device)
@@ -73,15 +80,15 @@ def test_factor_and_solve_entry_point(N, m, dtype, device):
 
 
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
-@pytest.mark.parametrize("N,m", [(40, 4), (9, 12)])
-def test_factor_and_solve_launch_graph(N, m, dtype, device):
+@pytest.mark.parametrize("n,N,m", [(32, 40, 4), (32, 9, 12), (128, 64, 1), (64, 30, 3)])
+def test_factor_and_solve_launch_graph(n, N, m, dtype, device):
     """The launch object replayed three times from restored inputs."""
     import torch
 
     from socu_amd.block_tridiag_solver import create_cholesky_factor_and_solve_launch
 
     tdt = getattr(torch, dtype)
-    n, B = 32, 2
+    B = 2
     L0, E0, b0 = _stack(n, N, m, B, seed=5 * N + m)
     L = torch.tensor(L0, dtype=tdt, device=device)
     E = torch.tensor(E0, dtype=tdt, device=device)
