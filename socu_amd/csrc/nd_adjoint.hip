@@ -72,7 +72,8 @@ __global__ __launch_bounds__(256) void k_adjoint_scatter(const T* __restrict__ u
   const long long e = blk - N;  // E block index when !isL
   T* out = isL ? dL + ((size_t)b * N + blk) * nn : dE + ((size_t)b * nE + e) * nn;
   const int tid = threadIdx.x;
-  const int tr = tid / 8, tc8 = tid % 8;  // thread owns row r0+tr, cols c0+tc8+8q
+  const int tc = tid & 31, tr = tid >> 5;  // thread owns column c0+tc, rows r0+tr+8q: each store
+                                          // instruction writes two whole 32-wide rows
   if (masked_out(mask, b)) return;  // masked systems keep their outputs
   const bool fill = MODE == kPsi && !isL && e >= N - 1;
   T acc[4] = {T(0), T(0), T(0), T(0)};
@@ -111,29 +112,41 @@ __global__ __launch_bounds__(256) void k_adjoint_scatter(const T* __restrict__ u
     const bool two = src.R != nullptr;
     for (int k0 = 0; k0 < m; k0 += CH) {
       const int kc = min(CH, m - k0);
+      // unconditional loads from clamped addresses, all four in flight
+      // before any is used, then the masking (a select on each load let the
+      // compiler give every load a branch and a wait of its own)
+      const T* Rp = two ? src.R : src.P;
+      const T* Sp = two ? src.S : src.Q;
       for (int idx = tid; idx < kAdjTile * CH; idx += 256) {
         const int i = idx / CH, k = idx % CH;
+        const int kk = min(k, kc - 1);
+        const size_t ro = (size_t)min(r0 + i, n - 1) * m + k0 + kk, co = (size_t)min(c0 + i, n - 1) * m + k0 + kk;
+        T pv = src.P[ro], qv = src.Q[co], rv = Rp[ro], sv = Sp[co];
+        pin_reg(pv);
+        pin_reg(qv);
+        pin_reg(rv);
+        pin_reg(sv);
         const bool okr = r0 + i < n && k < kc, okc = c0 + i < n && k < kc;
-        P[i][k] = okr ? src.P[(size_t)(r0 + i) * m + k0 + k] : T(0);
-        Q[i][k] = okc ? src.Q[(size_t)(c0 + i) * m + k0 + k] : T(0);
-        R[i][k] = okr && two ? src.R[(size_t)(r0 + i) * m + k0 + k] : T(0);
-        S[i][k] = okc && two ? src.S[(size_t)(c0 + i) * m + k0 + k] : T(0);
+        P[i][k] = okr ? pv : T(0);
+        Q[i][k] = okc ? qv : T(0);
+        R[i][k] = okr && two ? rv : T(0);
+        S[i][k] = okc && two ? sv : T(0);
       }
       __syncthreads();
       for (int k = 0; k < kc; ++k) {
-        const T pv = P[tr][k], rv = R[tr][k];
+        const T qv = Q[tc][k], sv = S[tc][k];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] += pv * Q[tc8 + 8 * j][k] + rv * S[tc8 + 8 * j][k];
+        for (int j = 0; j < 4; ++j) acc[j] += P[tr + 8 * j][k] * qv + R[tr + 8 * j][k] * sv;
       }
       __syncthreads();
     }
   }
-  const int r = r0 + tr;
-  if (r < n) {
+  const int c = c0 + tc;
+  if (c < n) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int c = c0 + tc8 + 8 * j;
-      if (c >= n) continue;
+      const int r = r0 + tr + 8 * j;
+      if (r >= n) continue;
       T val = fill ? T(0) : -acc[j];
       if (isL) val = (r > c) ? val : ((r == c) ? (MODE == kPsi ? T(0.5) * val : val) : T(0));
       out[(size_t)r * n + c] = val;

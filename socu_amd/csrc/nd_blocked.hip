@@ -1505,52 +1505,82 @@ __global__ __launch_bounds__(kThreads) void k_apply_bwd(const T* __restrict__ L,
   }
 }
 
-// One backward-sweep level for n == 32, one workgroup per (node, 16
-// right-hand sides): k_sweep_update<bwd> and k_apply_bwd of the node in one
-// launch,
+// One sweep step for n == 32 in one launch, one workgroup per (block, 16
+// right-hand sides), both products on 16x16 MFMA tiles (waves 0 and 1 take
+// rows 0-15 and 16-31, the right-hand sides are the 16 columns).
+// Backward (FWD = false), level l, node i (the reference's backward
+// substitution, block_tridiag_solver.py:581-643; k_sweep_update<bwd> and
+// k_apply_bwd in one):
 //   t   = x_i - R_i^T x_{i+s} - Lc_i x_{i-s}     (R_i = block (i+s, i) when
 //         i + s < N, Lc_i = block (i, i-s) when i >= s),
 //   x_i = Linv_i^T t = t / L_i[r][r] + U t       (U: the transposed inverse in
-//         the strict upper triangle of L_i),
-// (the reference's backward substitution, block_tridiag_solver.py:581-643).
+//         the strict upper triangle of L_i).
+// Forward (FWD = true), step l, target j = 2s(u+1)-1 (the reference's forward
+// substitution, block_tridiag_solver.py:425-485; k_sweep_update<fwd> of step
+// l and, for the targets that are level-(l+1) nodes (u even), k_apply_fwd of
+// level l+1 in one):
+//   t   = y_j - R_{j-s} y_{j-s} - Lc_{j+s}^T y_{j+s}   (R_{j-s} = block
+//         (j, j-s), Lc_{j+s} = block (j+s, j) when j + s < N),
+//   y_j = t, or Linv_j t = t / L_j[r][r] + U^T t for u even.
 // All three blocks and the three right-hand-side panels are loaded at once
-// and staged in LDS (32 x 33 tiles; panels of pitch rhs_pitch(m)); both
-// products run on 16x16 MFMA tiles, waves 0 and 1 taking rows 0-15 and
-// 16-31 with the right-hand sides as the 16 columns.  Columns at or past w
-// read whatever follows in LDS (a guard keeps those reads in the
-// allocation): they only reach output columns that are not stored.
+// and staged in LDS (32 x 33 tiles; panels of pitch rhs_pitch(m)).  Columns
+// at or past w read whatever follows in LDS (a guard keeps those reads in
+// the allocation): they only reach output columns that are not stored.
 template <typename T>
-constexpr size_t bwd32_lds(int m) {
+constexpr size_t sweep32_lds(int m) {
   return sizeof(T) * (3 * 32 * 33 + 3 * 32 * (size_t)rhs_pitch(m) + 16 + 32);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_bwd32(const T* __restrict__ L, const T* __restrict__ E,
-                                                   T* __restrict__ x, const uint8_t* __restrict__ mask, int N, int m,
-                                                   int nE, int s, int off) {
+template <typename T, bool FWD>
+__global__ __launch_bounds__(kThreads) void k_sweep32(const T* __restrict__ L, const T* __restrict__ E,
+                                                     T* __restrict__ x, const uint8_t* __restrict__ mask, int N, int m,
+                                                     int nE, int s, int off, int apply) {
   constexpr int n = 32, P = 33;
   constexpr size_t nn = 32 * 32;
   const int b = blockIdx.z;
   if (masked_out(mask, b)) return;
-  const int i = s - 1 + 2 * s * blockIdx.x;
   const int c0 = blockIdx.y * kRhsChunk;
   const int w = min(kRhsChunk, m - c0);
   const int yp = rhs_pitch(m);
-  const bool hasR = i + s < N, hasC = i >= s;
   const size_t nm = (size_t)n * m;
-  const T* U = L + ((size_t)b * N + i) * nn;
   const T* Eb = E + (size_t)b * nE * nn;
-  const T* R = hasR ? Eb + (size_t)(off + i / s) * nn : U;  // (U: a valid dummy address)
-  const T* Lc = hasC ? Eb + (size_t)(off + (i - s) / s) * nn : U;
   T* xb = x + (size_t)b * N * nm;
+  // Blocks used transposed (A(i, k) = M[k][i]) with panel Xp, and as they
+  // stand (A(i, k) = M[i][k]) with panel Xm.
+  int i;
+  bool hasT, hasS, inv;
+  const T* MT;
+  const T* MS;
+  const T* xp;
+  const T* xm;
+  if (!FWD) {
+    i = s - 1 + 2 * s * blockIdx.x;
+    hasT = i + s < N;  // R_i
+    hasS = i >= s;     // Lc_i
+    inv = true;
+  } else {
+    const int u = blockIdx.x;
+    i = 2 * s * (u + 1) - 1;
+    hasT = i + s < N;  // Lc_{j+s}
+    hasS = true;       // R_{j-s}
+    inv = apply != 0 && (u & 1) == 0;
+  }
+  const T* U = L + ((size_t)b * N + i) * nn;
   T* xi = xb + (size_t)i * nm;
-  const T* xp = hasR ? xb + (size_t)(i + s) * nm : xi;
-  const T* xm = hasC ? xb + (size_t)(i - s) * nm : xi;
+  if (!FWD) {
+    MT = hasT ? Eb + (size_t)(off + i / s) * nn : U;  // (U: a valid dummy address)
+    MS = hasS ? Eb + (size_t)(off + (i - s) / s) * nn : U;
+  } else {
+    MT = hasT ? Eb + (size_t)(off + i / s) * nn : U;
+    MS = Eb + (size_t)(off + (i - s) / s) * nn;
+  }
+  xp = hasT ? xb + (size_t)(i + s) * nm : xi;
+  xm = hasS ? xb + (size_t)(i - s) * nm : xi;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* Us = reinterpret_cast<T*>(smem_raw);  // 32 x 33 each
-  T* Rs = Us + 32 * P;
-  T* Cs = Rs + 32 * P;
-  T* Xp = Cs + 32 * P;  // 32 x yp each; Xp later holds t
+  T* Ts = Us + 32 * P;
+  T* Ss = Ts + 32 * P;
+  T* Xp = Ss + 32 * P;  // 32 x yp each; Xp later holds t
   T* Xm = Xp + 32 * yp;
   T* Xi = Xm + 32 * yp;
   T* dg = Xi + 32 * yp + 16;  // 1 / L[r][r] (after a 16-entry guard)
@@ -1558,13 +1588,13 @@ __global__ __launch_bounds__(kThreads) void k_bwd32(const T* __restrict__ L, con
   // every load at once (unconditional, from valid addresses), then the LDS
   // stores with the absent operands zeroed
   constexpr int PT = 32 * 32 / kThreads;
-  T uv[PT], rv[PT], cv[PT];
+  T uv[PT], tv[PT], sv[PT];
 #pragma unroll
   for (int e = 0; e < PT; ++e) {
     const int idx = tid + e * kThreads;
     uv[e] = U[idx];
-    rv[e] = R[idx];
-    cv[e] = Lc[idx];
+    tv[e] = MT[idx];
+    sv[e] = MS[idx];
   }
   // right-hand-side entries q = tid + kThreads e (row q / w, column q % w)
   constexpr int PQ = 32 * kRhsChunk / kThreads;
@@ -1580,35 +1610,41 @@ __global__ __launch_bounds__(kThreads) void k_bwd32(const T* __restrict__ L, con
 #pragma unroll
   for (int e = 0; e < PT; ++e) {
     pin_reg(uv[e]);
-    pin_reg(rv[e]);
-    pin_reg(cv[e]);
+    pin_reg(tv[e]);
+    pin_reg(sv[e]);
   }
 #pragma unroll
   for (int e = 0; e < PT; ++e) {
     const int idx = tid + e * kThreads;
     const int r = idx >> 5, c = idx & 31;
     Us[r * P + c] = uv[e];
-    Rs[r * P + c] = hasR ? rv[e] : T(0);
-    Cs[r * P + c] = hasC ? cv[e] : T(0);
+    Ts[r * P + c] = hasT ? tv[e] : T(0);
+    Ss[r * P + c] = hasS ? sv[e] : T(0);
     if (r == c) dg[r] = T(1) / uv[e];
   }
 #pragma unroll
   for (int e = 0; e < PQ; ++e) {
     const int q = tid + e * kThreads, qr = q / w, qc = q - qr * w;
     if (q < 32 * w) {
-      Xp[qr * yp + qc] = hasR ? xpv[e] : T(0);
-      Xm[qr * yp + qc] = hasC ? xmv[e] : T(0);
+      Xp[qr * yp + qc] = hasT ? xpv[e] : T(0);
+      Xm[qr * yp + qc] = hasS ? xmv[e] : T(0);
       Xi[qr * yp + qc] = xiv[e];
     }
   }
   __syncthreads();
-  // t = x_i - R^T xp - Lc xm: wave rt (0, 1) forms rows 16 rt .. 16 rt + 15
-  // (A(i, k) = R[k][16 rt + i] and Lc[16 rt + i][k]; B = the panels)
+  // t = x_i - MT^T xp - MS xm: wave rt (0, 1) forms rows 16 rt .. 16 rt + 15
   const int rt = wave;
   typename Mfma<T>::acc_t acc = acc_zero<T>();
   if (rt < 2) {
-    wave_gemm16<T>(acc, Rs + 16 * rt, 1, P, Xp, yp, 1, 32);
-    wave_gemm16<T>(acc, Cs + 16 * rt * P, P, 1, Xm, yp, 1, 32);
+    wave_gemm16<T>(acc, Ts + 16 * rt, 1, P, Xp, yp, 1, 32);
+    wave_gemm16<T>(acc, Ss + 16 * rt * P, P, 1, Xm, yp, 1, 32);
+  }
+  if (!inv) {
+    if (rt < 2)
+      acc16_foreach<T>(acc, [&](int r, int c, T v) {
+        if (c < w) xi[(size_t)(16 * rt + r) * m + c0 + c] = Xi[(16 * rt + r) * yp + c] - v;
+      });
+    return;
   }
   __syncthreads();  // every read of Xp done
   if (rt < 2)
@@ -1616,15 +1652,16 @@ __global__ __launch_bounds__(kThreads) void k_bwd32(const T* __restrict__ L, con
       if (c < w) Xp[(16 * rt + r) * yp + c] = Xi[(16 * rt + r) * yp + c] - v;
     });
   __syncthreads();
-  // x_i = A t with A(r, k) = U[r][k] (k > r), 1 / L[r][r] (k = r), 0 (k < r)
+  // backward: x_i = A t, A(r, k) = U[r][k] (k > r), 1 / L[r][r] (k = r);
+  // forward: y_j = A t, A(r, k) = U[k][r] (k < r), 1 / L[r][r] (k = r)
   if (rt < 2) {
     acc = acc_zero<T>();
     const int ii = l & 15, kq = l >> 4, row = 16 * rt + ii;
 #pragma unroll
     for (int k = 0; k < 32; k += 4) {
       const int kk = k + kq;
-      const T u = Us[row * P + kk];
-      const T a = (kk > row) ? u : ((kk == row) ? dg[row] : T(0));
+      const T u = FWD ? Us[kk * P + row] : Us[row * P + kk];
+      const T a = (FWD ? kk < row : kk > row) ? u : ((kk == row) ? dg[row] : T(0));
       acc = Mfma<T>::mma(a, Xp[kk * yp + ii], acc);
     }
     acc16_foreach<T>(acc, [&](int r, int c, T v) {
@@ -2019,6 +2056,21 @@ static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* m
   const int mch = (m + kRhsChunk - 1) / kRhsChunk;
   if (B <= 0 || mch <= 0) return hipSuccess;
   kt_begin(kKcForward, st);
+  if constexpr (NF == 32) {
+    // step l's updates and level l+1's applies in one launch (k_sweep32)
+    if (update && step >= 0) {
+      const Level lv = level_info(N, step);
+      const int items = N / (2 * lv.s);
+      if (items > 0) {
+        const size_t lds = sweep32_lds<T>(m);
+        const int ap = (apply_next && level_info(N, step + 1).dim > 0) ? 1 : 0;
+        hipLaunchKernelGGL((k_sweep32<T, true>), dim3(items, mch, B), dim3(kThreads), lds, st, L, E, x, mask, N, m,
+                           nE, lv.s, lv.off, ap);
+      }
+      kt_end(kKcForward, st);
+      return hipGetLastError();
+    }
+  }
   if (update && step >= 0) {
     const Level lv = level_info(N, step);
     const int items = N / (2 * lv.s);
@@ -2092,9 +2144,9 @@ static hipError_t backward_level_nb(const T* L, const T* E, T* x, const uint8_t*
   if (lv.dim <= 0 || B <= 0 || mch <= 0) return hipSuccess;
   kt_begin(kKcBackward, st);
   if constexpr (NF == 32) {  // update and apply in one launch
-    const size_t lds = bwd32_lds<T>(m);
-    hipLaunchKernelGGL(k_bwd32<T>, dim3(lv.dim, mch, B), dim3(kThreads), lds, st, L, E, x, mask, N, m, nE, lv.s,
-                       lv.off);
+    const size_t lds = sweep32_lds<T>(m);
+    hipLaunchKernelGGL((k_sweep32<T, false>), dim3(lv.dim, mch, B), dim3(kThreads), lds, st, L, E, x, mask, N, m, nE,
+                       lv.s, lv.off, 1);
     kt_end(kKcBackward, st);
     return hipGetLastError();
   }
