@@ -914,7 +914,9 @@ int batched_layout(socu_batched* h) {
 // Launches every non-empty class: the costliest on st, the others on side
 // streams forked from and joined back into st, so their workgroups fill the
 // machine together.
-int batched_enqueue(socu_batched* h, const unsigned char* mask, hipStream_t st) {
+int batched_enqueue(socu_batched* h, const unsigned char* mask, hipStream_t st,
+                    const socu::BatchProblem* probs = nullptr) {
+  if (probs == nullptr) probs = h->dprobs;
   int first = -1, nside = 0;
   hipError_t e = hipSuccess;
   for (int q = 0; q < kClasses && e == hipSuccess; ++q) {
@@ -927,13 +929,13 @@ int batched_enqueue(socu_batched* h, const unsigned char* mask, hipStream_t st) 
     if (nside == 0) e = hipEventRecord(h->fork, st);
     if (e == hipSuccess) e = hipStreamWaitEvent(h->side[nside], h->fork, 0);
     if (e == hipSuccess)
-      e = socu::batched_launch_class(h->dprobs, h->dorder + h->starts[c], h->tile[c], h->counts[c], h->mmax[c],
+      e = socu::batched_launch_class(probs, h->dorder + h->starts[c], h->tile[c], h->counts[c], h->mmax[c],
                                      h->waves[c], mask, h->op, h->dtype, h->side[nside]);
     if (e == hipSuccess) e = hipEventRecord(h->join[nside], h->side[nside]);
     ++nside;
   }
   if (e == hipSuccess && first >= 0)
-    e = socu::batched_launch_class(h->dprobs, h->dorder + h->starts[first], h->tile[first], h->counts[first],
+    e = socu::batched_launch_class(probs, h->dorder + h->starts[first], h->tile[first], h->counts[first],
                                    h->mmax[first], h->waves[first], mask, h->op, h->dtype, st);
   for (int k = 0; k < nside && e == hipSuccess; ++k) e = hipStreamWaitEvent(st, h->join[k], 0);
   if (e != hipSuccess) return hip_fail(e, "batched launch");
@@ -942,7 +944,11 @@ int batched_enqueue(socu_batched* h, const unsigned char* mask, hipStream_t st) 
 
 // For each size bucket, times every admissible (tile class >= the bucket's
 // own, wavefronts 1, 2 or 4) on a device copy of the batch and keeps the
-// fastest.  The caller's arrays are only read.  Needs batched_layout first.
+// fastest.  Then, with several buckets, the buckets run together (on forked
+// streams, as executed): the per-bucket picks and the same tiles with one
+// workgroup width for every bucket are timed as a whole and the fastest is
+// kept (buckets that share the machine favour other widths than buckets run
+// alone).  The caller's arrays are only read.  Needs batched_layout first.
 int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   tune_cache_load();
@@ -1028,9 +1034,62 @@ int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) 
     if (e == hipSuccess) {
       h->tile[b] = best_c;
       h->waves[b] = best_w;
-      tune_cache_store(keys[b], 8 * best_w + best_c);
     }
   }
+  int nonempty = 0;
+  for (int b = 0; b < kClasses; ++b) nonempty += h->counts[b] > 0;
+  if (e == hipSuccess && nonempty > 1) {
+    int pick[kClasses];
+    for (int b = 0; b < kClasses; ++b) pick[b] = h->waves[b];
+    float best = 1e30f;
+    int best_set[kClasses];
+    for (int b = 0; b < kClasses; ++b) best_set[b] = pick[b];
+    for (int cand = 0; cand < 4 && e == hipSuccess; ++cand) {  // the picks, then widths 1, 2, 4 throughout
+      for (int b = 0; b < kClasses; ++b) {
+        const int wv = cand == 0 ? pick[b] : (1 << (cand - 1));
+        h->waves[b] = socu::batched_fits(h->tile[b], wv, h->mmax[b], h->dtype) ? wv : pick[b];
+      }
+      // captured in a graph, as it runs (launched eagerly on three streams
+      // the widths rank differently), and the median of five runs (the whole
+      // batch differs by a few percent between widths, about the run-to-run
+      // spread of one run)
+      hipGraph_t g = nullptr;
+      hipGraphExec_t gx = nullptr;
+      hipStream_t cs = nullptr;
+      e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        const int erc = batched_enqueue(h, nullptr, cs, dcopy);
+        const hipError_t e2 = hipStreamEndCapture(cs, &g);
+        e = erc != SOCU_OK ? hipErrorUnknown : e2;
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+      float runs[5];
+      for (int rep = 0; rep < 5 && e == hipSuccess; ++rep) {
+        e = socu::batched_copy(h->dprobs, dcopy, h->count, h->op, h->dtype, ts);
+        if (e == hipSuccess) e = hipEventRecord(e0, ts);
+        if (e == hipSuccess) e = hipGraphLaunch(gx, ts);
+        if (e == hipSuccess) e = hipEventRecord(e1, ts);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        runs[rep] = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&runs[rep], e0, e1);
+      }
+      if (gx) (void)hipGraphExecDestroy(gx);
+      if (g) (void)hipGraphDestroy(g);
+      if (cs) (void)hipStreamDestroy(cs);
+      if (e == hipSuccess) {
+        std::sort(runs, runs + 5);
+        if (runs[2] < best) {
+          best = runs[2];
+          for (int b = 0; b < kClasses; ++b) best_set[b] = h->waves[b];
+        }
+      }
+    }
+    for (int b = 0; b < kClasses; ++b) h->waves[b] = best_set[b];
+  }
+  if (e == hipSuccess)
+    for (int b = 0; b < kClasses; ++b)
+      if (h->counts[b] > 0) tune_cache_store(keys[b], 8 * h->waves[b] + h->tile[b]);
   if (e != hipSuccess) rc = hip_fail(e, "batched autotune");
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
@@ -1108,8 +1167,9 @@ int socu_batched_create(socu_batched** out, int op, const socu_batch_system* sys
   rc = batched_layout(h);
   if (rc == SOCU_OK && !tune) {
     // SOCU_BATCHED_WAVES=1|2|4 sets the wavefronts per workgroup of an
-    // untuned plan (every variant of the kernels can then be run on purpose)
-    // (or the widest that fits the bucket's right-hand sides in LDS)
+    // untuned plan (every variant of the kernels can then be run on purpose;
+    // default 1), narrowed where its LDS does not fit the bucket's
+    // right-hand sides
     const char* w = getenv("SOCU_BATCHED_WAVES");
     const int wv = w ? atoi(w) : 1;
     if (wv == 1 || wv == 2 || wv == 4)
