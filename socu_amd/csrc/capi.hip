@@ -944,11 +944,7 @@ int batched_enqueue(socu_batched* h, const unsigned char* mask, hipStream_t st,
 
 // For each size bucket, times every admissible (tile class >= the bucket's
 // own, wavefronts 1, 2 or 4) on a device copy of the batch and keeps the
-// fastest.  Then, with several buckets, the buckets run together (on forked
-// streams, as executed): the per-bucket picks and the same tiles with one
-// workgroup width for every bucket are timed as a whole and the fastest is
-// kept (buckets that share the machine favour other widths than buckets run
-// alone).  The caller's arrays are only read.  Needs batched_layout first.
+// fastest.  The caller's arrays are only read.  Needs batched_layout first.
 int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   tune_cache_load();
@@ -1035,57 +1031,6 @@ int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) 
       h->tile[b] = best_c;
       h->waves[b] = best_w;
     }
-  }
-  int nonempty = 0;
-  for (int b = 0; b < kClasses; ++b) nonempty += h->counts[b] > 0;
-  if (e == hipSuccess && nonempty > 1) {
-    int pick[kClasses];
-    for (int b = 0; b < kClasses; ++b) pick[b] = h->waves[b];
-    float best = 1e30f;
-    int best_set[kClasses];
-    for (int b = 0; b < kClasses; ++b) best_set[b] = pick[b];
-    for (int cand = 0; cand < 4 && e == hipSuccess; ++cand) {  // the picks, then widths 1, 2, 4 throughout
-      for (int b = 0; b < kClasses; ++b) {
-        const int wv = cand == 0 ? pick[b] : (1 << (cand - 1));
-        h->waves[b] = socu::batched_fits(h->tile[b], wv, h->mmax[b], h->dtype) ? wv : pick[b];
-      }
-      // captured in a graph, as it runs (launched eagerly on three streams
-      // the widths rank differently), and the median of five runs (the whole
-      // batch differs by a few percent between widths, about the run-to-run
-      // spread of one run)
-      hipGraph_t g = nullptr;
-      hipGraphExec_t gx = nullptr;
-      hipStream_t cs = nullptr;
-      e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
-      if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-      if (e == hipSuccess) {
-        const int erc = batched_enqueue(h, nullptr, cs, dcopy);
-        const hipError_t e2 = hipStreamEndCapture(cs, &g);
-        e = erc != SOCU_OK ? hipErrorUnknown : e2;
-      }
-      if (e == hipSuccess) e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
-      float runs[5];
-      for (int rep = 0; rep < 5 && e == hipSuccess; ++rep) {
-        e = socu::batched_copy(h->dprobs, dcopy, h->count, h->op, h->dtype, ts);
-        if (e == hipSuccess) e = hipEventRecord(e0, ts);
-        if (e == hipSuccess) e = hipGraphLaunch(gx, ts);
-        if (e == hipSuccess) e = hipEventRecord(e1, ts);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        runs[rep] = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&runs[rep], e0, e1);
-      }
-      if (gx) (void)hipGraphExecDestroy(gx);
-      if (g) (void)hipGraphDestroy(g);
-      if (cs) (void)hipStreamDestroy(cs);
-      if (e == hipSuccess) {
-        std::sort(runs, runs + 5);
-        if (runs[2] < best) {
-          best = runs[2];
-          for (int b = 0; b < kClasses; ++b) best_set[b] = h->waves[b];
-        }
-      }
-    }
-    for (int b = 0; b < kClasses; ++b) h->waves[b] = best_set[b];
   }
   if (e == hipSuccess)
     for (int b = 0; b < kClasses; ++b)
