@@ -1670,6 +1670,120 @@ __global__ __launch_bounds__(kThreads) void k_sweep32(const T* __restrict__ L, c
   }
 }
 
+// The same sweep step for the other block sizes n <= 128 (NB = 1..4; for
+// NB >= 2 three staged blocks would not fit LDS): the block operands are read
+// from global memory
+// in the MFMA operand layout (each element is used once: the right-hand
+// sides are the 16 columns of every product), the three panels and the
+// intermediate t are staged in LDS.  Four wavefronts take the NP/16 row tiles
+// in turn; the triangular apply skips the zero 16-column blocks of Linv.
+// Unconditional loads from clamped addresses, masked after.
+template <typename T, int NB>
+constexpr size_t sweepnb_lds(int m) {
+  return sizeof(T) * (4 * (size_t)(32 * NB) * rhs_pitch(m) + 16 + 32 * NB);
+}
+
+template <typename T, int NB, int NF, bool FWD>
+__global__ __launch_bounds__(kThreads) void k_sweep_nb(const T* __restrict__ L, const T* __restrict__ E,
+                                                      T* __restrict__ x, const uint8_t* __restrict__ mask, int N,
+                                                      int n_, int m, int nE, int s, int off, int apply) {
+  const int n = NF ? NF : n_;
+  constexpr int NP = 32 * NB, RT = NP / 16;
+  const int b = blockIdx.z;
+  if (masked_out(mask, b)) return;
+  const int c0 = blockIdx.y * kRhsChunk;
+  const int w = min(kRhsChunk, m - c0);
+  const int yp = rhs_pitch(m);
+  const size_t nn = (size_t)n * n, nm = (size_t)n * m;
+  const T* Eb = E + (size_t)b * nE * nn;
+  T* xb = x + (size_t)b * N * nm;
+  int i;
+  bool hasT, hasS, inv;
+  if (!FWD) {
+    i = s - 1 + 2 * s * blockIdx.x;
+    hasT = i + s < N;  // R_i, used transposed
+    hasS = i >= s;     // Lc_i
+    inv = true;
+  } else {
+    const int u = blockIdx.x;
+    i = 2 * s * (u + 1) - 1;
+    hasT = i + s < N;  // Lc_{j+s}, used transposed
+    hasS = true;       // R_{j-s}
+    inv = apply != 0 && (u & 1) == 0;
+  }
+  const T* U = L + ((size_t)b * N + i) * nn;
+  T* xi = xb + (size_t)i * nm;
+  const T* MT = hasT ? Eb + (size_t)(off + i / s) * nn : U;  // (U: a valid dummy address)
+  const T* MS = hasS ? Eb + (size_t)(off + (i - s) / s) * nn : U;
+  const T* xp = hasT ? xb + (size_t)(i + s) * nm : xi;
+  const T* xm = hasS ? xb + (size_t)(i - s) * nm : xi;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Xp = reinterpret_cast<T*>(smem_raw);  // NP x yp each
+  T* Xm = Xp + NP * yp;
+  T* Xi = Xm + NP * yp;
+  T* Tt = Xi + NP * yp;
+  T* dg = Tt + NP * yp + 16;  // 1 / L[r][r] (after a 16-entry guard)
+  const int tid = threadIdx.x, l = lane_id(), wave = wave_id();
+  for (int q = tid; q < NP * w; q += kThreads) {
+    const int k = q / w, c = q - k * w;
+    const size_t xo = (size_t)min(k, n - 1) * m + c0 + c;
+    const T vp = xp[xo], vm = xm[xo], vi = xi[xo];
+    Xp[k * yp + c] = (hasT && k < n) ? vp : T(0);
+    Xm[k * yp + c] = (hasS && k < n) ? vm : T(0);
+    Xi[k * yp + c] = (k < n) ? vi : T(0);
+  }
+  for (int r = tid; r < NP; r += kThreads) {
+    const T d = U[(size_t)min(r, n - 1) * (n + 1)];
+    dg[r] = (r < n) ? T(1) / d : T(1);
+  }
+  __syncthreads();
+  const int li = l & 15, kq = l >> 4;
+  const int jb = min(li, w - 1);
+  // t = x_i - MT^T xp - MS xm
+  for (int t = wave; t < RT; t += kThreads / 64) {
+    const int ii = 16 * t + li, ic = min(ii, n - 1);
+    typename Mfma<T>::acc_t acc = acc_zero<T>();
+#pragma unroll 4
+    for (int k = 0; k < NP; k += 4) {
+      const int kk = k + kq, kc = min(kk, n - 1);
+      const T a1 = MT[(size_t)kc * n + ic];
+      const T a2 = MS[(size_t)ic * n + kc];
+      const bool ok = kk < n && ii < n;
+      acc = Mfma<T>::mma((hasT && ok) ? a1 : T(0), Xp[kk * yp + jb], acc);
+      acc = Mfma<T>::mma((hasS && ok) ? a2 : T(0), Xm[kk * yp + jb], acc);
+    }
+    acc16_foreach<T>(acc, [&](int r, int c, T v) {
+      const int row = 16 * t + r;
+      if (c < w) {
+        const T tv = Xi[row * yp + c] - v;
+        if (inv) Tt[row * yp + c] = tv;
+        else if (row < n) xi[(size_t)row * m + c0 + c] = tv;
+      }
+    });
+  }
+  if (!inv) return;
+  __syncthreads();
+  // backward: x_i = A t, A(r, k) = U[r][k] (k > r), 1 / L[r][r] (k = r);
+  // forward: y_j = A t, A(r, k) = U[k][r] (k < r), 1 / L[r][r] (k = r)
+  for (int t = wave; t < RT; t += kThreads / 64) {
+    const int ii = 16 * t + li, ic = min(ii, n - 1);
+    const int kb = FWD ? 0 : 16 * t, ke = FWD ? 16 * (t + 1) : NP;
+    typename Mfma<T>::acc_t acc = acc_zero<T>();
+#pragma unroll 4
+    for (int k = kb; k < ke; k += 4) {
+      const int kk = k + kq, kc = min(kk, n - 1);
+      const T u = FWD ? U[(size_t)kc * n + ic] : U[(size_t)ic * n + kc];
+      const bool strict = FWD ? kk < ii : kk > ii;
+      const T a = (kk < n && ii < n) ? (strict ? u : ((kk == ii) ? dg[ii] : T(0))) : T(0);
+      acc = Mfma<T>::mma(a, Tt[kk * yp + jb], acc);
+    }
+    acc16_foreach<T>(acc, [&](int r, int c, T v) {
+      const int row = 16 * t + r;
+      if (row < n && c < w) xi[(size_t)row * m + c0 + c] = v;
+    });
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1723,8 +1837,32 @@ static hipError_t set_blk_attrs() {
   e = hipFuncSetAttribute((const void*)k_trsm_node<T, NB, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)trsm_node_lds<T, NB>());
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)k_trsm_node<T, NB, 32 * NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)trsm_node_lds<T, NB>());
+  e = hipFuncSetAttribute((const void*)k_trsm_node<T, NB, 32 * NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)trsm_node_lds<T, NB>());
+  const int lsw = (int)sweepnb_lds<T, NB>(kRhsChunk);  // the widest right-hand-side chunk
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_sweep_nb<T, NB, 0, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lsw);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_sweep_nb<T, NB, 0, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lsw);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_sweep_nb<T, NB, 32 * NB, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lsw);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_sweep_nb<T, NB, 32 * NB, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lsw);
+  return e;
+}
+
+// Whether the level sweeps of blocks other than n = 32 run as one fused
+// kernel per step (k_sweep_nb) instead of update and apply kernels;
+// SOCU_SWEEP_NB=0 selects the separate kernels (for measurement).
+static bool sweep_nb_enabled() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("SOCU_SWEEP_NB");
+    v = (e != nullptr && atoi(e) == 0) ? 0 : 1;
+  }
+  return v == 1;
 }
 
 // Whether a level's trsm takes k_trsm_node, and with how many chunks per
@@ -2056,20 +2194,25 @@ static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* m
   const int mch = (m + kRhsChunk - 1) / kRhsChunk;
   if (B <= 0 || mch <= 0) return hipSuccess;
   kt_begin(kKcForward, st);
-  if constexpr (NF == 32) {
-    // step l's updates and level l+1's applies in one launch (k_sweep32)
-    if (update && step >= 0) {
-      const Level lv = level_info(N, step);
-      const int items = N / (2 * lv.s);
-      if (items > 0) {
+  // step l's updates and level l+1's applies in one launch (k_sweep32,
+  // k_sweep_nb)
+  if (update && step >= 0 && (NF == 32 || sweep_nb_enabled())) {
+    const Level lv = level_info(N, step);
+    const int items = N / (2 * lv.s);
+    if (items > 0) {
+      const int ap = (apply_next && level_info(N, step + 1).dim > 0) ? 1 : 0;
+      if constexpr (NF == 32) {
         const size_t lds = sweep32_lds<T>(m);
-        const int ap = (apply_next && level_info(N, step + 1).dim > 0) ? 1 : 0;
         hipLaunchKernelGGL((k_sweep32<T, true>), dim3(items, mch, B), dim3(kThreads), lds, st, L, E, x, mask, N, m,
                            nE, lv.s, lv.off, ap);
+      } else {
+        const size_t lds = sweepnb_lds<T, NB>(m);
+        hipLaunchKernelGGL((k_sweep_nb<T, NB, NF, true>), dim3(items, mch, B), dim3(kThreads), lds, st, L, E, x, mask,
+                           N, n, m, nE, lv.s, lv.off, ap);
       }
-      kt_end(kKcForward, st);
-      return hipGetLastError();
     }
+    kt_end(kKcForward, st);
+    return hipGetLastError();
   }
   if (update && step >= 0) {
     const Level lv = level_info(N, step);
@@ -2149,6 +2292,14 @@ static hipError_t backward_level_nb(const T* L, const T* E, T* x, const uint8_t*
                        lv.s, lv.off, 1);
     kt_end(kKcBackward, st);
     return hipGetLastError();
+  } else {
+    if (sweep_nb_enabled()) {
+      const size_t lds = sweepnb_lds<T, NB>(m);
+      hipLaunchKernelGGL((k_sweep_nb<T, NB, NF, false>), dim3(lv.dim, mch, B), dim3(kThreads), lds, st, L, E, x, mask,
+                         N, n, m, nE, lv.s, lv.off, 1);
+      kt_end(kKcBackward, st);
+      return hipGetLastError();
+    }
   }
   // the top node has no neighbours (i = s-1, i+s >= N)
   if (!(lv.dim == 1 && lv.s - 1 + lv.s >= N)) {
