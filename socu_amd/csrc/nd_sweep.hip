@@ -310,9 +310,19 @@ __global__ __launch_bounds__((SweepShape<T, NB>::THREADS)) void k_sweep(const T*
   __shared__ int s_task, s_ok;
   const int t = threadIdx.x;
 
+  // With every system masked out (a refinement step after convergence) the
+  // launch ends at its first barrier; the mask is read while the first ticket
+  // is taken.  (Masked tasks touch no state, so neither does this exit.)
+  bool first = true;
   for (;;) {
     if (t == 0) s_task = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
+    int any = 1;
+    if (first && mask != nullptr) {
+      any = 0;
+      for (int b = t; b < B; b += S::THREADS) any |= masked_out(mask, b) ? 0 : 1;
+    }
+    first = false;
+    if (!__syncthreads_or(any)) break;
     int l = 0, q = 0;
     bool bwd = false;
     const int task = __builtin_amdgcn_readfirstlane(s_task);  // uniform: scalar decode and addressing

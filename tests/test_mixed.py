@@ -89,6 +89,35 @@ def test_mixed_needs_fewer_than_max_iter_for_well_conditioned(device):
     assert rr[0] <= 1e-10 and its[0] <= 3, (rr, its)
 
 
+@pytest.mark.parametrize("B", [1, 3])
+@pytest.mark.parametrize("graph", [False, True])
+def test_mixed_iterations_after_convergence_leave_x_unchanged(B, graph, device):
+    """Refinement steps past convergence (every system converged: the
+    persistent sweep ends at once) must not touch x: the result of a solve
+    capped at the number of steps it needs is reproduced bit for bit with a
+    higher cap, on repeated calls."""
+    import torch
+    from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
+
+    L0, E0, b0 = _problem(128, 64, 1, 16.0, seed=3, B=B)
+    Lt, Et = torch.tensor(L0, device=device), torch.tensor(E0, device=device)
+    x_need = torch.tensor(b0, device=device)
+    probe = create_cholesky_mixed_factor_and_solve_launch(Lt, Et, x_need, max_iter=10, use_cuda_graph=graph)
+    probe()
+    rr, its = probe.stats()
+    need = int(max(its))
+    assert need < 10 and max(rr) <= 1e-10, (rr, its)
+    for cap in (need, need + 3):
+        x = torch.tensor(b0, device=device)
+        run = create_cholesky_mixed_factor_and_solve_launch(Lt, Et, x, max_iter=cap, use_cuda_graph=graph)
+        for _ in range(2):
+            x.copy_(torch.tensor(b0, device=device))
+            run()
+            torch.cuda.synchronize()
+            assert torch.equal(x, x_need), cap
+            assert [int(v) for v in run.stats()[1]] == [int(v) for v in its]
+
+
 def test_mixed_non_spd_stops_without_hanging(device):
     import torch
     from socu_amd.mixed import create_cholesky_mixed_factor_and_solve_launch
