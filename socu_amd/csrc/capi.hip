@@ -944,7 +944,8 @@ int batched_enqueue(socu_batched* h, const unsigned char* mask, hipStream_t st,
 
 // For each size bucket, times every admissible (tile class >= the bucket's
 // own, wavefronts 1, 2 or 4) on a device copy of the batch and keeps the
-// fastest.  The caller's arrays are only read.  Needs batched_layout first.
+// fastest, near-ties going to the wider workgroup.  The caller's arrays are
+// only read.  Needs batched_layout first.
 int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   tune_cache_load();
@@ -953,7 +954,7 @@ int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) 
   for (int b = 0; b < kClasses; ++b) {
     if (h->counts[b] == 0) continue;
     char buf[160];
-    snprintf(buf, sizeof buf, "batched3:%s:op%d:bucket%d:count%d:cost%lld", h->dtype == SOCU_DTYPE_F64 ? "f64" : "f32",
+    snprintf(buf, sizeof buf, "batched4:%s:op%d:bucket%d:count%d:cost%lld", h->dtype == SOCU_DTYPE_F64 ? "f64" : "f32",
              h->op, b, h->counts[b], h->bcost[b]);
     keys[b] = buf;
     auto it = g_tune.find(keys[b]);
@@ -1004,11 +1005,15 @@ int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) 
         (g_tune.count(keys[b]) && (g_tune[keys[b]] & 7) >= b &&
          socu::batched_fits(g_tune[keys[b]] & 7, g_tune[keys[b]] >> 3, h->mmax[b], h->dtype)))
       continue;
-    float best = 1e30f;
-    int best_c = b, best_w = 1;
+    struct Cand {
+      int c, w;
+      float ms;
+    };
+    std::vector<Cand> cands;
     for (int c = b; c < kClasses && e == hipSuccess; ++c) {
       for (int w : {1, 2, 4}) {
         if (!socu::batched_fits(c, w, h->mmax[b], h->dtype)) continue;
+        cands.push_back({c, w, 1e30f});
         for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
           e = socu::batched_copy(h->dprobs, dcopy, h->count, h->op, h->dtype, ts);
           if (e == hipSuccess) e = hipEventRecord(e0, ts);
@@ -1019,17 +1024,27 @@ int batched_tune(socu_batched* h, const std::vector<socu::BatchProblem>& probs) 
           if (e == hipSuccess) e = hipEventSynchronize(e1);
           float ms = 0.f;
           if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-          if (e == hipSuccess && ms < best) {
-            best = ms;
-            best_c = c;
-            best_w = w;
-          }
+          if (e == hipSuccess && ms < cands.back().ms) cands.back().ms = ms;
         }
       }
     }
-    if (e == hipSuccess) {
-      h->tile[b] = best_c;
-      h->waves[b] = best_w;
+    // Isolated times within kTuneTie of the fastest are ties, settled for
+    // the most wavefronts per system: a wider workgroup runs more of a
+    // system's tasks at once, which shortens the bucket's tail when the
+    // buckets share the GPU in the real launch (measured on the 4096-system
+    // MPC batch: the isolated picks of one and two wavefronts differ by 1-2%,
+    // and two wavefronts for the large buckets make the whole batch 3-5%
+    // faster).
+    if (e == hipSuccess && !cands.empty()) {
+      constexpr float kTuneTie = 0.02f;
+      float tmin = 1e30f;
+      for (const Cand& k : cands) tmin = std::min(tmin, k.ms);
+      const Cand* pick = nullptr;
+      for (const Cand& k : cands)
+        if (k.ms <= tmin * (1.f + kTuneTie) && (pick == nullptr || k.w > pick->w || (k.w == pick->w && k.ms < pick->ms)))
+          pick = &k;
+      h->tile[b] = pick->c;
+      h->waves[b] = pick->w;
     }
   }
   if (e == hipSuccess)
