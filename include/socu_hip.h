@@ -247,11 +247,13 @@ int socu_profile_op(int op, void* L, void* E, void* x, const unsigned char* mask
  * workgroups of 1, 2 or 4 wavefronts, the launches on forked streams.  With
  * `tune` the create call times every admissible (tile size, wavefronts) per
  * bucket on a device copy of the batch (the caller's arrays are only read)
- * and keeps the fastest; results are cached per process and, if the
+ * and keeps the fastest (times within 2% of it are ties, settled for the
+ * more wavefronts); results are cached per process and, if the
  * environment variable SOCU_TUNE_CACHE names a file, appended there and
  * reused by later processes.  Without `tune` each bucket uses its own bound
- * and one wavefront (or the count in the environment variable
- * SOCU_BATCHED_WAVES: 1, 2 or 4).  create copies the system table to the device and
+ * and two wavefronts (or the count in the environment variable
+ * SOCU_BATCHED_WAVES: 1, 2 or 4), fewer where the LDS does not fit.  create
+ * copies the system table to the device and
  * synchronises; execute allocates nothing and synchronises nothing.
  * socu_batched_config returns per bucket the system count, tile size and
  * wavefronts per workgroup. */
