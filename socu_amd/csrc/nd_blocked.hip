@@ -17,9 +17,10 @@
 //                :202-228).
 //   k_schur_blk  one workgroup per 64x64 output tile, MFMA, owner computes:
 //                L_j -= R R^T + Lc^T Lc,  F -= R Lc (syrk.py, gemm.py).
-//   k_sweep_update, k_apply_fwd, k_apply_bwd  forward step / backward level:
-//                row-split block-times-panel updates, then triangular GEMVs
-//                with the stored inverses (replaces K9-K16,
+//   k_sweep32, k_sweep_nb, k_apply_fwd  the sweeps, one launch per step and
+//                direction: the block-times-panel update and the triangular
+//                apply with the stored inverse in one kernel (k_apply_fwd:
+//                the first forward level) (replaces K9-K16,
 //                block_tridiag_solver.py:425-736).
 #include <stdlib.h>
 
@@ -1075,7 +1076,7 @@ __device__ __forceinline__ void panel_store(T* P, const T* v, bool cols, int i0,
 
 // With x (optional; n x m blocks of right-hand sides) the diagonal-tile
 // tasks of a target j also apply level l's forward-sweep update to rows
-// 32 tr .. of y_j (k_sweep_update<fwd>):
+// 32 tr .. of y_j (the forward sweep's update):
 //   y_j -= R_{j-s} y_{j-s} + Lc_{j+s}^T y_{j+s},
 // with the operand panels the Schur product has staged: rows 32 tr.. of R and
 // columns 32 tr.. of Lc.  The nodes' y must have had their level-l step
@@ -1260,120 +1261,18 @@ __global__ __launch_bounds__(kThreads) void k_schur_small(T* __restrict__ L, T* 
 // ---------------------------------------------------------------------------
 // solves, with the explicit inverses of the diagonal blocks (see k_potrf_blk)
 //
-// Forward step l (levels ascending):
-//   k_sweep_update<fwd>: for every receiving block j = 2s(u+1)-1 and every
-//       32-row block rb:  y_j[rb] -= R_{j-s}[rb,:] y_{j-s} + Lc_{j+s}[:,rb]^T y_{j+s}
-//   k_apply_fwd:         for every node j of level l+1:  y_j <- Linv_j y_j
-// Backward level l (levels descending):
-//   k_sweep_update<bwd>: for every node i and 32-row block rb:
-//                         x_i[rb] -= R_i[:,rb]^T x_{i+s} + Lc_i[rb,:] x_{i-s}
-//   k_apply_bwd:         x_i <- Linv_i^T x_i
-// The updates spread each block over NB workgroups (one per 32 rows) so that
-// the E blocks stream through many CUs; the applies are triangular GEMVs.
+// Forward step l (levels ascending), k_sweep32 / k_sweep_nb:
+//   for every receiving block j = 2s(u+1)-1:  y_j -= R_{j-s} y_{j-s} + Lc_{j+s}^T y_{j+s},
+//   then, for the nodes j of level l+1 among them:  y_j <- Linv_j y_j
+//   (k_apply_fwd: the applies of level 0, which no update precedes)
+// Backward level l (levels descending), k_sweep32 / k_sweep_nb:
+//   for every node i:  x_i <- Linv_i^T (x_i - R_i^T x_{i+s} - Lc_i x_{i-s})
 // ---------------------------------------------------------------------------
 // Right-hand-side panels in LDS have pitch yp = min(m, kRhsChunk) + 1
 // (dynamic shared memory sized by the launch: few right-hand sides leave room
 // for more workgroups per compute unit).
 __host__ __device__ constexpr int rhs_pitch(int m) { return (m < kRhsChunk ? m : kRhsChunk) + 1; }
 
-// One 32-row block of the update.  "Row operand" A (rows r0..r0+31 of an
-// n x n block, times Ya) and "column operand" C (columns r0..r0+31, i.e.
-// C[:, rb]^T, times Yc); either may be absent.
-template <typename T, int NB, int NF, bool BWD>
-__global__ __launch_bounds__(kThreads) void k_sweep_update(const T* __restrict__ E, T* __restrict__ x,
-                                                          const uint8_t* __restrict__ mask, int N, int n_, int m,
-                                                          int nE, int s, int off) {
-  const int yp = rhs_pitch(m);
-  const int n = NF ? NF : n_;
-  constexpr int NP = 32 * NB;
-  constexpr int AP = NP + 1;
-  const int b = blockIdx.z;
-  if (masked_out(mask, b)) return;
-  const int rb = blockIdx.y % NB, ch = blockIdx.y / NB;
-  const int r0 = 32 * rb;
-  if (r0 >= n) return;
-  const int c0 = ch * kRhsChunk;
-  const int w = min(kRhsChunk, m - c0);
-  const size_t nn = (size_t)n * n, nm = (size_t)n * m;
-  const T* Eb = E + (size_t)b * nE * nn;
-  T* xb = x + (size_t)b * N * nm;
-
-  int j, ja = -1, jc = -1;
-  const T* Aop = nullptr;  // rows r0.. used as A[r][k]
-  const T* Cop = nullptr;  // columns r0.. used as C[k][r]
-  if (!BWD) {
-    j = 2 * s * (blockIdx.x + 1) - 1;
-    ja = j - s;
-    Aop = Eb + (size_t)(off + (j - s) / s) * nn;  // R of node j-s: block (j, j-s)
-    if (j + s < N) {
-      jc = j + s;
-      Cop = Eb + (size_t)(off + j / s) * nn;  // Lc of node j+s: block (j+s, j)
-    }
-  } else {
-    j = s - 1 + 2 * s * blockIdx.x;
-    if (j >= s) {
-      ja = j - s;
-      Aop = Eb + (size_t)(off + (j - s) / s) * nn;  // Lc of node j: block (j, j-s)
-    }
-    if (j + s < N) {
-      jc = j + s;
-      Cop = Eb + (size_t)(off + j / s) * nn;  // R of node j: block (j+s, j)
-    }
-    if (ja < 0 && jc < 0) return;
-  }
-
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* Ya = reinterpret_cast<T*>(smem_raw);  // NP x yp
-  T* Yc = Ya + NP * yp;                     // NP x yp
-  __shared__ T As[32 * AP];
-  __shared__ T Cs[NP * 33];
-  const int tid = threadIdx.x;
-  constexpr int PA = 32 * NP / kThreads;  // elements per thread of a 32 x NP slab
-  T av[PA], cv[PA];
-#pragma unroll
-  for (int e = 0; e < PA; ++e) {
-    const int idx = tid + e * kThreads;
-    const int ra = idx / NP, ka = idx - ra * NP;  // A[r0+ra][ka]
-    const int kc = idx >> 5, rc = idx & 31;       // C[kc][r0+rc]
-    T ga = T(0), gc = T(0);
-    if (Aop) ga = Aop[(size_t)min(r0 + ra, n - 1) * n + min(ka, n - 1)];
-    if (Cop) gc = Cop[(size_t)min(kc, n - 1) * n + min(r0 + rc, n - 1)];
-    av[e] = (Aop && r0 + ra < n && ka < n) ? ga : T(0);
-    cv[e] = (Cop && kc < n && r0 + rc < n) ? gc : T(0);
-  }
-  for (int q = tid; q < NP * w; q += kThreads) {
-    const int k = q / w, c = q - k * w;
-    Ya[k * yp + c] = (ja >= 0 && k < n) ? xb[(size_t)ja * nm + (size_t)k * m + c0 + c] : T(0);
-    Yc[k * yp + c] = (jc >= 0 && k < n) ? xb[(size_t)jc * nm + (size_t)k * m + c0 + c] : T(0);
-  }
-#pragma unroll
-  for (int e = 0; e < PA; ++e) {
-    const int idx = tid + e * kThreads;
-    As[(idx / NP) * AP + (idx % NP)] = av[e];
-    Cs[(idx >> 5) * 33 + (idx & 31)] = cv[e];
-  }
-  __syncthreads();
-
-  // 8 lanes per output (row r, column c), each summing NP/8 of K, then a
-  // width-8 shuffle reduction.
-  constexpr int KP = NP / 8;
-  const int part = tid & 7;
-  for (int q = tid >> 3; q < 32 * w; q += kThreads / 8) {
-    const int r = q & 31, c = q >> 5;
-    T acc = T(0);
-#pragma unroll
-    for (int e = 0; e < KP; ++e) {
-      const int k = part * KP + e;
-      acc += As[r * AP + k] * Ya[k * yp + c] + Cs[k * 33 + r] * Yc[k * yp + c];
-    }
-#pragma unroll
-    for (int o = 4; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 8);
-    if (part == 0 && r0 + r < n) {
-      T* dst = xb + (size_t)j * nm + (size_t)(r0 + r) * m + c0 + c;
-      *dst -= acc;
-    }
-  }
-}
 
 // y_j <- Linv_j y_j for the nodes j = s-1+2s*t of one level.
 // out[r] = y[r] / L[r][r] + sum_{k<r} U[k][r] y[k]; thread (r, part) reads
@@ -1443,82 +1342,20 @@ __global__ __launch_bounds__(kThreads) void k_apply_fwd(const T* __restrict__ L,
   }
 }
 
-// x_i <- Linv_i^T x_i for the nodes of one level.
-// out[r] = x[r] / L[r][r] + sum_{k>r} U[r][k] x[k]: 8 lanes per row read
-// contiguous slices of row r of U, then a width-8 shuffle reduction.
-template <typename T, int NB, int NF>
-__global__ __launch_bounds__(kThreads) void k_apply_bwd(const T* __restrict__ L, T* __restrict__ x,
-                                                       const uint8_t* __restrict__ mask, int N, int n_, int m,
-                                                       int s) {
-  const int n = NF ? NF : n_;
-  constexpr int NP = 32 * NB;
-  constexpr int KPL = NP / 8;       // contiguous elements per lane
-  constexpr int PASSES = NP / 32;   // 32 rows per pass (8 lanes each)
-  const int b = blockIdx.z;
-  if (masked_out(mask, b)) return;
-  const int i = s - 1 + 2 * s * blockIdx.x;
-  const int c0 = blockIdx.y * kRhsChunk;
-  const int w = min(kRhsChunk, m - c0);
-  const size_t nn = (size_t)n * n, nm = (size_t)n * m;
-  const T* U = L + ((size_t)b * N + i) * nn;
-  T* y = x + ((size_t)b * N + i) * nm;
-  const int yp = rhs_pitch(m);
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* Y = reinterpret_cast<T*>(smem_raw);  // NP x yp
-  const int tid = threadIdx.x;
-  const int sub = tid & 7, rr = tid >> 3;
-  T a[PASSES][KPL];
-#pragma unroll
-  for (int p = 0; p < PASSES; ++p)
-#pragma unroll
-    for (int e = 0; e < KPL; ++e) a[p][e] = U[(size_t)min(rr + 32 * p, n - 1) * n + min(sub * KPL + e, n - 1)];
-#pragma unroll
-  for (int p = 0; p < PASSES; ++p)
-#pragma unroll
-    for (int e = 0; e < KPL; ++e) pin_reg(a[p][e]);  // all loads in flight before any use
-#pragma unroll
-  for (int p = 0; p < PASSES; ++p) {
-    const int r = rr + 32 * p;
-#pragma unroll
-    for (int e = 0; e < KPL; ++e) {
-      const int k = sub * KPL + e;
-      a[p][e] = (r < n && k < n && k > r) ? a[p][e] : T(0);
-    }
-  }
-  for (int q = tid; q < NP * w; q += kThreads) {
-    const int k = q / w, c = q - k * w;
-    Y[k * yp + c] = (k < n) ? y[(size_t)k * m + c0 + c] : T(0);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < PASSES; ++p) {
-    const int r = rr + 32 * p;
-    const T dg = (sub == 0 && r < n) ? U[(size_t)r * n + r] : T(1);
-    for (int c = 0; c < w; ++c) {
-      T acc = T(0);
-#pragma unroll
-      for (int e = 0; e < KPL; ++e) acc += a[p][e] * Y[(sub * KPL + e) * yp + c];
-#pragma unroll
-      for (int o = 4; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 8);
-      if (sub == 0 && r < n) y[(size_t)r * m + c0 + c] = acc + Y[r * yp + c] / dg;
-    }
-  }
-}
 
 // One sweep step for n == 32 in one launch, one workgroup per (block, 16
 // right-hand sides), both products on 16x16 MFMA tiles (waves 0 and 1 take
 // rows 0-15 and 16-31, the right-hand sides are the 16 columns).
 // Backward (FWD = false), level l, node i (the reference's backward
-// substitution, block_tridiag_solver.py:581-643; k_sweep_update<bwd> and
-// k_apply_bwd in one):
+// substitution, block_tridiag_solver.py:581-643; update and apply in one):
 //   t   = x_i - R_i^T x_{i+s} - Lc_i x_{i-s}     (R_i = block (i+s, i) when
 //         i + s < N, Lc_i = block (i, i-s) when i >= s),
 //   x_i = Linv_i^T t = t / L_i[r][r] + U t       (U: the transposed inverse in
 //         the strict upper triangle of L_i).
 // Forward (FWD = true), step l, target j = 2s(u+1)-1 (the reference's forward
-// substitution, block_tridiag_solver.py:425-485; k_sweep_update<fwd> of step
-// l and, for the targets that are level-(l+1) nodes (u even), k_apply_fwd of
-// level l+1 in one):
+// substitution, block_tridiag_solver.py:425-485; the update of step l and,
+// for the targets that are level-(l+1) nodes (u even), the apply of level
+// l+1 in one):
 //   t   = y_j - R_{j-s} y_{j-s} - Lc_{j+s}^T y_{j+s}   (R_{j-s} = block
 //         (j, j-s), Lc_{j+s} = block (j+s, j) when j + s < N),
 //   y_j = t, or Linv_j t = t / L_j[r][r] + U^T t for u even.
@@ -1789,16 +1626,8 @@ __global__ __launch_bounds__(kThreads) void k_sweep_nb(const T* __restrict__ L, 
 // ---------------------------------------------------------------------------
 // dynamic LDS of the sweep kernels for m right-hand sides
 template <typename T, int NB>
-constexpr size_t update_lds(int m) {
-  return sizeof(T) * 2 * (size_t)(32 * NB) * rhs_pitch(m);
-}
-template <typename T, int NB>
 constexpr size_t apply_fwd_lds(int m) {
   return sizeof(T) * (1 + kThreads / (32 * NB)) * (size_t)(32 * NB) * rhs_pitch(m);
-}
-template <typename T, int NB>
-constexpr size_t apply_bwd_lds(int m) {
-  return sizeof(T) * (size_t)(32 * NB) * rhs_pitch(m);
 }
 
 template <typename T, int NB>
@@ -1851,18 +1680,6 @@ static hipError_t set_blk_attrs() {
     e = hipFuncSetAttribute((const void*)k_sweep_nb<T, NB, 32 * NB, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             lsw);
   return e;
-}
-
-// Whether the level sweeps of blocks other than n = 32 run as one fused
-// kernel per step (k_sweep_nb) instead of update and apply kernels;
-// SOCU_SWEEP_NB=0 selects the separate kernels (for measurement).
-static bool sweep_nb_enabled() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("SOCU_SWEEP_NB");
-    v = (e != nullptr && atoi(e) == 0) ? 0 : 1;
-  }
-  return v == 1;
 }
 
 // Whether a level's trsm takes k_trsm_node, and with how many chunks per
@@ -2196,7 +2013,7 @@ static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* m
   kt_begin(kKcForward, st);
   // step l's updates and level l+1's applies in one launch (k_sweep32,
   // k_sweep_nb)
-  if (update && step >= 0 && (NF == 32 || sweep_nb_enabled())) {
+  if (update && step >= 0) {
     const Level lv = level_info(N, step);
     const int items = N / (2 * lv.s);
     if (items > 0) {
@@ -2213,15 +2030,6 @@ static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* m
     }
     kt_end(kKcForward, st);
     return hipGetLastError();
-  }
-  if (update && step >= 0) {
-    const Level lv = level_info(N, step);
-    const int items = N / (2 * lv.s);
-    if (items > 0) {
-      const size_t lds = update_lds<T, NB>(m);
-      hipLaunchKernelGGL((k_sweep_update<T, NB, NF, false>), dim3(items, NB * mch, B), dim3(kThreads), lds, st, E, x,
-                         mask, N, n, m, nE, lv.s, lv.off);
-    }
   }
   const Level nx = level_info(N, step + 1);
   if (apply_next && nx.dim > 0) {
@@ -2293,25 +2101,12 @@ static hipError_t backward_level_nb(const T* L, const T* E, T* x, const uint8_t*
     kt_end(kKcBackward, st);
     return hipGetLastError();
   } else {
-    if (sweep_nb_enabled()) {
-      const size_t lds = sweepnb_lds<T, NB>(m);
-      hipLaunchKernelGGL((k_sweep_nb<T, NB, NF, false>), dim3(lv.dim, mch, B), dim3(kThreads), lds, st, L, E, x, mask,
-                         N, n, m, nE, lv.s, lv.off, 1);
-      kt_end(kKcBackward, st);
-      return hipGetLastError();
-    }
+    const size_t lds = sweepnb_lds<T, NB>(m);
+    hipLaunchKernelGGL((k_sweep_nb<T, NB, NF, false>), dim3(lv.dim, mch, B), dim3(kThreads), lds, st, L, E, x, mask, N,
+                       n, m, nE, lv.s, lv.off, 1);
+    kt_end(kKcBackward, st);
+    return hipGetLastError();
   }
-  // the top node has no neighbours (i = s-1, i+s >= N)
-  if (!(lv.dim == 1 && lv.s - 1 + lv.s >= N)) {
-    const size_t lds = update_lds<T, NB>(m);
-    hipLaunchKernelGGL((k_sweep_update<T, NB, NF, true>), dim3(lv.dim, NB * mch, B), dim3(kThreads), lds, st, E, x,
-                       mask, N, n, m, nE, lv.s, lv.off);
-  }
-  const size_t lds_apply = apply_bwd_lds<T, NB>(m);
-  hipLaunchKernelGGL((k_apply_bwd<T, NB, NF>), dim3(lv.dim, mch, B), dim3(kThreads), lds_apply, st, L, x, mask, N, n,
-                     m, lv.s);
-  kt_end(kKcBackward, st);
-  return hipGetLastError();
 }
 
 template <typename T>

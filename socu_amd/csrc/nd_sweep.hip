@@ -1,7 +1,7 @@
 // Persistent dataflow solve sweeps for one right-hand side (m = 1), n <= 128.
 //
-// The level-synchronous sweeps (k_sweep_update / k_apply_fwd / k_apply_bwd)
-// need two launches per level and direction, each waiting for the slowest
+// The level-synchronous sweeps (k_sweep32 / k_sweep_nb / k_apply_fwd)
+// need a launch per level and direction, each waiting for the slowest
 // workgroup of its level and paying a dispatch gap; at the upper levels, where
 // a level is one or two nodes, those gaps are the whole cost.  Here one launch
 // runs both sweeps as node tasks that wait on per-node counters:
