@@ -1512,20 +1512,39 @@ __global__ __launch_bounds__(kThreads) void k_sweep32(const T* __restrict__ L, c
 // from global memory
 // in the MFMA operand layout (each element is used once: the right-hand
 // sides are the 16 columns of every product), the three panels and the
-// intermediate t are staged in LDS.  Four wavefronts take the NP/16 row tiles
-// in turn; the triangular apply skips the zero 16-column blocks of Linv.
+// intermediate t are staged in LDS; the triangular apply skips the zero
+// 16-column blocks of Linv.  Four wavefronts take the NP/16 row tiles in turn,
+// except in the FP64 backward sweep (sweepnb_wide): one wavefront per tile,
+// which issues every block load of its tile, the apply's included, before its
+// first MFMA.  Measured against the four-wavefront loop: the FP64 backward
+// sweeps gain (n=64 N=1024 m=16 factor_and_solve 0.612 -> 0.605 ms, n=128
+// N=512 m=8 1.160 -> 1.151 ms); the forward sweep, which runs beside the
+// factor, slowed the factor's kernels more than it gained, and the FP32
+// backward sweeps of the mixed solver's correction solves were slower.
 // Unconditional loads from clamped addresses, masked after.
+template <typename T, bool FWD>
+constexpr bool sweepnb_wide() {
+  return !FWD && sizeof(T) == 8;
+}
+
+template <typename T, int NB, bool FWD>
+constexpr int sweepnb_threads() {
+  return sweepnb_wide<T, FWD>() ? 64 * 2 * NB : kThreads;
+}
+
 template <typename T, int NB>
 constexpr size_t sweepnb_lds(int m) {
   return sizeof(T) * (4 * (size_t)(32 * NB) * rhs_pitch(m) + 16 + 32 * NB);
 }
 
 template <typename T, int NB, int NF, bool FWD>
-__global__ __launch_bounds__(kThreads) void k_sweep_nb(const T* __restrict__ L, const T* __restrict__ E,
-                                                      T* __restrict__ x, const uint8_t* __restrict__ mask, int N,
-                                                      int n_, int m, int nE, int s, int off, int apply) {
+__global__ __launch_bounds__((sweepnb_threads<T, NB, FWD>())) void k_sweep_nb(const T* __restrict__ L,
+                                                                        const T* __restrict__ E, T* __restrict__ x,
+                                                                        const uint8_t* __restrict__ mask, int N,
+                                                                        int n_, int m, int nE, int s, int off,
+                                                                        int apply) {
   const int n = NF ? NF : n_;
-  constexpr int NP = 32 * NB, RT = NP / 16;
+  constexpr int NP = 32 * NB, RT = NP / 16, KS = NP / 4, kT = sweepnb_threads<T, NB, FWD>();
   const int b = blockIdx.z;
   if (masked_out(mask, b)) return;
   const int c0 = blockIdx.y * kRhsChunk;
@@ -1561,23 +1580,81 @@ __global__ __launch_bounds__(kThreads) void k_sweep_nb(const T* __restrict__ L, 
   T* Tt = Xi + NP * yp;
   T* dg = Tt + NP * yp + 16;  // 1 / L[r][r] (after a 16-entry guard)
   const int tid = threadIdx.x, l = lane_id(), wave = wave_id();
-  for (int q = tid; q < NP * w; q += kThreads) {
-    const int k = q / w, c = q - k * w;
-    const size_t xo = (size_t)min(k, n - 1) * m + c0 + c;
-    const T vp = xp[xo], vm = xm[xo], vi = xi[xo];
-    Xp[k * yp + c] = (hasT && k < n) ? vp : T(0);
-    Xm[k * yp + c] = (hasS && k < n) ? vm : T(0);
-    Xi[k * yp + c] = (k < n) ? vi : T(0);
-  }
-  for (int r = tid; r < NP; r += kThreads) {
-    const T d = U[(size_t)min(r, n - 1) * (n + 1)];
-    dg[r] = (r < n) ? T(1) / d : T(1);
-  }
-  __syncthreads();
   const int li = l & 15, kq = l >> 4;
+  auto stage = [&]() {
+    for (int q = tid; q < NP * w; q += kT) {
+      const int k = q / w, c = q - k * w;
+      const size_t xo = (size_t)min(k, n - 1) * m + c0 + c;
+      const T vp = xp[xo], vm = xm[xo], vi = xi[xo];
+      Xp[k * yp + c] = (hasT && k < n) ? vp : T(0);
+      Xm[k * yp + c] = (hasS && k < n) ? vm : T(0);
+      Xi[k * yp + c] = (k < n) ? vi : T(0);
+    }
+    for (int r = tid; r < NP; r += kT) {
+      const T d = U[(size_t)min(r, n - 1) * (n + 1)];
+      dg[r] = (r < n) ? T(1) / d : T(1);
+    }
+  };
   const int jb = min(li, w - 1);
+  if constexpr (sweepnb_wide<T, FWD>()) {
+    // backward, one wavefront per 16-row tile t (32-bit element offsets from the
+    // wavefront-uniform block addresses)
+    const int t = wave;
+    const int ii = 16 * t + li;
+    const unsigned ic = min(ii, n - 1);
+    T a1[KS], a2[KS], a3[KS];
+#pragma unroll
+    for (int q = 0; q < KS; ++q) {
+      const unsigned kc = min(4 * q + kq, n - 1);
+      a1[q] = MT[kc * (unsigned)n + ic];
+      a2[q] = MS[ic * (unsigned)n + kc];
+    }
+    stage();
+    __syncthreads();
+    // t = x_i - R_i^T x_{i+s} - Lc_i x_{i-s}
+    typename Mfma<T>::acc_t acc = acc_zero<T>();
+#pragma unroll
+    for (int q = 0; q < KS; ++q) {
+      const int kk = 4 * q + kq;
+      pin_reg(a1[q]);
+      pin_reg(a2[q]);
+      const bool ok = kk < n && ii < n;
+      acc = Mfma<T>::mma((hasT && ok) ? a1[q] : T(0), Xp[kk * yp + jb], acc);
+      acc = Mfma<T>::mma((hasS && ok) ? a2[q] : T(0), Xm[kk * yp + jb], acc);
+    }
+    acc16_foreach<T>(acc, [&](int r, int c, T v) {
+      if (c < w) Tt[(16 * t + r) * yp + c] = Xi[(16 * t + r) * yp + c] - v;
+    });
+    // the apply's operands (k >= 16 t), loaded while the other wavefronts
+    // finish t; steps below 4 t load the entries of step 4 t again (the same
+    // addresses: no further traffic) and are not used
+#pragma unroll
+    for (int q = 0; q < KS; ++q) {
+      const unsigned kc = min(4 * max(q, 4 * t) + kq, n - 1);
+      a3[q] = U[ic * (unsigned)n + kc];
+    }
+    __syncthreads();
+    // x_i = A t, A(r, k) = U[r][k] (k > r), 1 / L[r][r] (k = r)
+    acc = acc_zero<T>();
+#pragma unroll
+    for (int q = 0; q < KS; ++q) {
+      if (q >= 4 * t) {  // wavefront-uniform
+        const int kk = 4 * q + kq;
+        pin_reg(a3[q]);
+        const T a = (kk < n && ii < n) ? ((kk > ii) ? a3[q] : ((kk == ii) ? dg[ii] : T(0))) : T(0);
+        acc = Mfma<T>::mma(a, Tt[kk * yp + jb], acc);
+      }
+    }
+    acc16_foreach<T>(acc, [&](int r, int c, T v) {
+      const int row = 16 * t + r;
+      if (row < n && c < w) xi[(size_t)row * m + c0 + c] = v;
+    });
+    return;
+  }
+  stage();
+  __syncthreads();
   // t = x_i - MT^T xp - MS xm
-  for (int t = wave; t < RT; t += kThreads / 64) {
+  for (int t = wave; t < RT; t += kT / 64) {
     const int ii = 16 * t + li, ic = min(ii, n - 1);
     typename Mfma<T>::acc_t acc = acc_zero<T>();
 #pragma unroll 4
@@ -1602,7 +1679,7 @@ __global__ __launch_bounds__(kThreads) void k_sweep_nb(const T* __restrict__ L, 
   __syncthreads();
   // backward: x_i = A t, A(r, k) = U[r][k] (k > r), 1 / L[r][r] (k = r);
   // forward: y_j = A t, A(r, k) = U[k][r] (k < r), 1 / L[r][r] (k = r)
-  for (int t = wave; t < RT; t += kThreads / 64) {
+  for (int t = wave; t < RT; t += kT / 64) {
     const int ii = 16 * t + li, ic = min(ii, n - 1);
     const int kb = FWD ? 0 : 16 * t, ke = FWD ? 16 * (t + 1) : NP;
     typename Mfma<T>::acc_t acc = acc_zero<T>();
@@ -2024,8 +2101,8 @@ static hipError_t forward_step_nb(const T* L, const T* E, T* x, const uint8_t* m
                            nE, lv.s, lv.off, ap);
       } else {
         const size_t lds = sweepnb_lds<T, NB>(m);
-        hipLaunchKernelGGL((k_sweep_nb<T, NB, NF, true>), dim3(items, mch, B), dim3(kThreads), lds, st, L, E, x, mask,
-                           N, n, m, nE, lv.s, lv.off, ap);
+        hipLaunchKernelGGL((k_sweep_nb<T, NB, NF, true>), dim3(items, mch, B), dim3(sweepnb_threads<T, NB, true>()), lds,
+                           st, L, E, x, mask, N, n, m, nE, lv.s, lv.off, ap);
       }
     }
     kt_end(kKcForward, st);
@@ -2102,8 +2179,8 @@ static hipError_t backward_level_nb(const T* L, const T* E, T* x, const uint8_t*
     return hipGetLastError();
   } else {
     const size_t lds = sweepnb_lds<T, NB>(m);
-    hipLaunchKernelGGL((k_sweep_nb<T, NB, NF, false>), dim3(lv.dim, mch, B), dim3(kThreads), lds, st, L, E, x, mask, N,
-                       n, m, nE, lv.s, lv.off, 1);
+    hipLaunchKernelGGL((k_sweep_nb<T, NB, NF, false>), dim3(lv.dim, mch, B), dim3(sweepnb_threads<T, NB, false>()), lds,
+                       st, L, E, x, mask, N, n, m, nE, lv.s, lv.off, 1);
     kt_end(kKcBackward, st);
     return hipGetLastError();
   }
